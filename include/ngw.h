@@ -1,0 +1,190 @@
+/*
+ * ngw.h -- C ABI of libngw, the gfx950 (MI355X) ISDF-Lanczos GW hot path.
+ *
+ * Drop-in boundary for NanoGW's Lanczos self-energy path.  Every entry point
+ * names the NanoGW routine (file:line) it replaces.  All arrays are Fortran
+ * column-major with 0-based C pointers; complex arrays are interleaved
+ * (re, im) pairs of double, i.e. Fortran complex(dp).  Energies are Rydberg
+ * (NanoGW's internal unit, e^2 = 2) unless an argument says eV.
+ *
+ * Pair ordering of transition-space vectors (length ncv = nval * ncond):
+ *   icv = iv + nval * ic   (0-based iv < nval, ic < ncond; band of ic is
+ *   nval + ic), the conduction-outer / valence-inner order NanoGW builds in
+ *   shared/setup_g.F90z:154-168.
+ *
+ * Host pointers: every entry point below takes HOST arrays, like the Fortran
+ * callers it replaces, except the *_dev entry points, which take device
+ * pointers and run on the context's stream.  The operator (Psi at the ISDF
+ * points, M, energies) stays resident on the GPU inside the context between
+ * calls.
+ *
+ * Errors: functions return NGW_OK (0) or a negative status; ngw_last_error()
+ * gives the message.  NanoGW reacts to errors with die() (shared/die.F90:10);
+ * the Fortran shim maps a non-zero status to die().
+ * Threading: one context is used by one host thread at a time.
+ */
+#ifndef NGW_H
+#define NGW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGW_OK 0
+#define NGW_ERR_ARG (-1)      /* invalid argument or size */
+#define NGW_ERR_STATE (-2)    /* operator not uploaded / inconsistent sizes */
+#define NGW_ERR_HIP (-3)      /* HIP runtime or hipFFT failure */
+#define NGW_ERR_NOMEM (-4)    /* device allocation failed */
+#define NGW_ERR_RANGE (-5)    /* polyfit_sqrt: xmin <= 1e-4 (NanoGW dies there) */
+
+#define NGW_PREC_F64 0   /* all GEMMs FP64 (the parity path) */
+#define NGW_PREC_F32 1   /* C x, M t and C^T w GEMMs in FP32, FP32 accumulate */
+#define NGW_PREC_BF16 2  /* C x and C^T w in BF16 with per-band Psi scaling, FP32 accumulate */
+
+typedef struct ngw_context ngw_context;
+
+/* Library version string and the last error message of this thread. */
+const char* ngw_version(void);
+const char* ngw_last_error(void);
+
+/* Context on HIP device `device` with its own stream. */
+int ngw_create(ngw_context** ctx, int device);
+int ngw_destroy(ngw_context* ctx);
+/* Wait for all work queued on the context's stream. */
+int ngw_synchronize(ngw_context* ctx);
+
+/*
+ * Upload the ISDF operator, replacing the fields of type(ISDF) the Lanczos
+ * path reads (shared/typedefs.F90:24-72):
+ *   psi_intp(ld_psi, nband)  Psi_intp_loc(:, 1:nband, 1, 1): orbitals at the
+ *                            n_intp ISDF points, bands 1..nval occupied
+ *   mmtrx(ld_m, n_intp)      Mmtrx_loc(:, :, 1, 1, 1, 1, 1): Coulomb kernel
+ *   e1(nband)                kpt%wfn(1,1)%e1 (Ry)
+ * All nval x (nband - nval) pairs form the transition space.
+ */
+int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double* psi_intp,
+                 int ld_psi, const double* mmtrx, int ld_m, const double* e1);
+
+/* GEMM precision of the operator (NGW_PREC_*); the Lanczos alpha/beta, the
+ * continued fraction, Sigma_c assembly and the QP Newton solve stay FP64. */
+int ngw_set_precision(ngw_context* ctx, int precision);
+
+/*
+ * Operator apply on blksz vectors, replaces matvec_isdf_BLOCK
+ * (shared/apply_matvec_BLOCK.F90:25) and, with tamm_d = 1, the TDA branch of
+ * matvec_isdf_lowcomsym (shared/apply_matvec_lowcomsym.F90:31, :396-403):
+ *   full: hvec = R^2 vec + 4 R^1/2 C^T M C R^1/2 vec
+ *   TDA : hvec = R vec + 2 C^T M C vec
+ * vec, hvec: (ncv, blksz).  ncv must equal nval*ncond.
+ */
+int ngw_matvec_isdf_block(ngw_context* ctx, const double* vec, double* hvec, int ncv,
+                          int blksz, int tamm_d);
+
+/*
+ * Horner p(H) vec, replaces polynomial_matvec_isdf_BLOCK
+ * (shared/apply_matvec_BLOCK.F90:210): polynomial(1..npoly+1) holds
+ * a_n .. a_0 (highest order first); npoly matvecs.
+ */
+int ngw_polynomial_matvec_isdf_block(ngw_context* ctx, const double* vec, double* hvec,
+                                     int ncv, const double* polynomial, int npoly, int blksz,
+                                     int tamm_d);
+
+/*
+ * blksz independent Lanczos recurrences on H_aux = p(H) (no
+ * reorthogonalisation) plus the continued fraction
+ * spectra(iz, j) = <v0_j| (zz(iz, j) - H_aux)^-1 |v0_j> / |v0_j|^2 ... as in
+ * lanczos_spectra_isdf_BLOCK (shared/lanczos_spectra_BLOCK.F90:7):
+ *   v0(ncv, blksz), v0_norm(blksz), zz(nz, blksz) complex,
+ *   spectra(nz, blksz) complex (output).
+ * a_elmt, b_elmt (niter, blksz) receive the recurrence coefficients when not
+ * NULL.  Also replaces lanczos_spectra_isdf (shared/lanczos_spectra.f90:7)
+ * with blksz = 1.
+ */
+int ngw_lanczos_spectra_isdf_block(ngw_context* ctx, const double* v0, const double* v0_norm,
+                                   int ncv, const double* zz, int nz, int niter,
+                                   const double* polynomial, int npoly, double* spectra,
+                                   int blksz, int tamm_d, double* a_elmt, double* b_elmt);
+
+/*
+ * Sigma_c(omega) of the Lanczos GW drivers for nqp QP states, replaces the
+ * GW body of Zcalculate_sigma_lanczos_BLOCK
+ * (SIGMA/calculate_sigma_lanczos_BLOCK.F90z:201-444; TDA increments from
+ * SIGMA/calculate_sigma_lanczos_lowcomsym.F90z:712-745):
+ *   qp_band(nqp)   0-based band index m of each QP state
+ *   nen, deltae    energy_data_points and energy_range (Ry); the grid is
+ *                  z_j = e1(m) - deltae/2 + j deltae/(nen-1)
+ *   ecuts          dynamic_energy_resolution gamma (Ry)
+ *   niter, polynomial(npoly+1)  lanczos_niter and the sqrt polynomial
+ *   sigma_c(nen, nqp) complex (Ry, output)
+ * Every (QP state, band) Lanczos run is done on the device in batched form;
+ * max_cols (0 = automatic) bounds how many runs share one batch.
+ */
+int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, double deltae,
+                      double ecuts, int niter, const double* polynomial, int npoly, int tamm_d,
+                      int max_cols, double* sigma_c);
+
+/*
+ * QP energies by Newton-Raphson on the cubic spline of
+ * f(E) = E_dft + Sigma_x + Sigma_c(E) - V_xc - E (eV), replaces the diagonal
+ * block of printsigma (SIGMA/printsigma.F90:97-128) with newtonr
+ * (SIGMA/newtonr.f90:18), spline_evaluate, spline and splint; run on the GPU
+ * in FP64, one state per thread.
+ *   nqp states, nen grid points, deltae_ev = energy_range in eV
+ *   edft(nqp) = e0*ryd, en0(nqp) = e1*ryd, sigx(nqp), vxc(nqp) real, eV
+ *   sigma_c(nen, nqp) complex, eV
+ * Outputs (nqp): eqp, zfac, sigma0 (complex), imsig, ierr (0 = converged).
+ */
+int ngw_qp_newton(ngw_context* ctx, int nqp, int nen, double deltae_ev, const double* edft,
+                  const double* en0, const double* sigx, const double* vxc,
+                  const double* sigma_c, double* eqp, double* zfac, double* sigma0,
+                  double* imsig, int* ierr);
+
+/*
+ * Least-squares monomial fit of sqrt(x) on [xmin, xmax] through nx nodes,
+ * replaces polyfit_sqrt (shared/polyfit.f90:1) and, with chebyshev = 1,
+ * polyfitcheb_sqrt (shared/polyfitcheb.f90:1); beta(npoly+1) is returned
+ * lowest order first (beta(1) = a_0) as in NanoGW; the Lanczos entry points
+ * take the reversed order.  Host-side, FP64.
+ */
+int ngw_polyfit_sqrt(double xmin, double xmax, int npoly, int nx, int chebyshev, double* beta,
+                     double* fit_error);
+
+/*
+ * ISDF Coulomb kernel M = (ntrans / h^3) zeta^T V zeta, replaces the V step
+ * of isdf_parallel_sym_UltraLowMem (shared/isdf_parallel_sym_UltraLowMem.F90:
+ * 686-875) with the isolated-system Poisson solve Zpoisson
+ * (shared/fft_module.F90z:122-177) and the truncated kernel
+ * v(G) = 8 pi (1 - cos(|G| R_c)) / |G|^2, v(0) = 8 pi R_c^2 / 2
+ * (Zcreate_coul_0D, shared/fft_module.F90z:628-691):
+ *   zeta(ld_zeta, n_intp)  interpolation vectors on the ngrid points
+ *   grid(3, ngrid)         integer grid coordinates in [-nfft/2, nfft/2)
+ *   nfft[3]                FFT box; h grid step; trunc = R_c
+ *   mmtrx(ld_m, n_intp)    output
+ * Batched R2C/C2R FFTs (hipFFT) with hand-written gather / kernel-multiply /
+ * scatter kernels and an FP64 MFMA GEMM for zeta^T (V zeta).
+ */
+int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, const double* zeta,
+                            long ld_zeta, const int* grid, const int* nfft, double h,
+                            double trunc, double ntrans, double* mmtrx, int ld_m);
+
+/*
+ * Device-pointer operator apply for callers that keep vectors on the GPU
+ * (the serving layer): same as ngw_matvec_isdf_block with d_vec, d_hvec
+ * device pointers of (ncv, blksz) with leading dimension ld.
+ */
+int ngw_matvec_isdf_block_dev(ngw_context* ctx, const double* d_vec, double* d_hvec, long ld,
+                              int blksz, int tamm_d);
+
+/*
+ * FP64 GEMM C(m,n) = A(m,k) B(k,n) on device pointers (column-major), the
+ * MFMA kernel the operator uses; exported for the library's own tests of
+ * the gfx950 matrix-core lane layout.
+ */
+int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, long lda,
+                     const double* d_b, long ldb, double* d_c, long ldc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGW_H */

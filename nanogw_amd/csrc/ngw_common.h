@@ -1,0 +1,71 @@
+// Shared definitions of the gfx950 ISDF-Lanczos library (libngw).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/ngw.h"
+
+namespace ngw {
+
+// Thread-local last-error text returned by ngw_last_error().
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define NGW_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::ngw::fail(NGW_ERR_HIP, std::string(#call " -> ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define NGW_TRY(call)                 \
+    do {                              \
+        int s_ = (call);              \
+        if (s_ != NGW_OK) return s_;  \
+    } while (0)
+
+// Device buffer that grows on demand and is freed with its owner.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Precision of the three GEMM stages of the operator.
+enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2 };
+
+}  // namespace ngw
+
+// Device-resident state of one ISDF operator (what type(ISDF) carries for the
+// Lanczos path): Psi at the ISDF points, the Coulomb kernel M, energies.
+struct ngw_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // problem
+    int n_intp = 0, nband = 0, nval = 0, ncond = 0;
+    long ncv = 0;
+    int precision = ngw::PREC_F64;
+    ngw::DevBuf psi;      // n_intp x nband, column-major, ld = ld_psi
+    int ld_psi = 0;
+    ngw::DevBuf mmat;     // n_intp x n_intp, ld = ld_m
+    int ld_m = 0;
+    ngw::DevBuf e1;       // nband energies (Ry)
+    ngw::DevBuf sqrtr;    // ncv: (e_c - e_v)^1/2
+    ngw::DevBuf diag_full;  // ncv: sqrtR^3   (applied to sqrtR*x)
+    ngw::DevBuf diag_tda;   // ncv: sqrtR^2   (applied to x)
+    // low-precision operator copies (FP32 / BF16 with per-band scales)
+    ngw::DevBuf psi32, m32, psibf, mbf, psi_scale, m_scale;
+    bool lowp_ready = false;
+    // workspaces
+    ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;
+    // host staging for drop-in calls
+    std::vector<double> hbuf;
+};

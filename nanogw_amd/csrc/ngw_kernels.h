@@ -1,0 +1,93 @@
+// Internal kernel interfaces of libngw.
+#pragma once
+
+#include "ngw_common.h"
+
+namespace ngw {
+
+// Epilogue of the operator's last GEMM (K3), applied per output element
+// (pair index idx of vector b, GEMM value Y = (C^T M C x')(idx)):
+//   hx  = d(idx) x'(idx) + beta s(idx) Y        (x' = s o x, or x if s == null)
+//   r   = alpha hx + gamma v0(idx)               (one Horner stage)
+//   out = acc(idx) + r                           (Lanczos w += p(H) q)
+struct OperatorEpilogue {
+    const double* x = nullptr; long ldx = 0;
+    const double* d = nullptr;
+    const double* s = nullptr;
+    double beta = 0.0, alpha = 1.0;
+    const double* v0 = nullptr; long ldv0 = 0; double gamma = 0.0;
+    const double* acc = nullptr; long ldacc = 0;
+    double* out = nullptr; long ldo = 0;
+
+    __device__ __forceinline__ void apply(long idx, int b, double Y) const
+    {
+        double sv = s ? s[idx] : 1.0;
+        double hx = (beta * sv) * Y;
+        if (x) {
+            double xv = x[idx + (long)b * ldx];
+            double xp = s ? sv * xv : xv;
+            hx = d[idx] * xp + hx;
+        }
+        double r = alpha * hx;
+        if (v0) r = r + gamma * v0[idx + (long)b * ldv0];
+        if (acc) r = acc[idx + (long)b * ldacc] + r;
+        out[idx + (long)b * ldo] = r;
+    }
+};
+
+// One operator application on nvec device vectors (the matvec of
+// apply_matvec_BLOCK.F90 with a fused Horner / Lanczos epilogue).
+struct OperatorArgs {
+    const double* x = nullptr; long ldx = 0;
+    int nvec = 0;
+    int tda = 0;
+    double* out = nullptr; long ldo = 0;
+    double alpha = 1.0;
+    const double* v0 = nullptr; long ldv0 = 0; double gamma = 0.0;
+    const double* acc = nullptr; long ldacc = 0;
+
+    OperatorEpilogue epilogue(const ngw_context* ctx) const
+    {
+        OperatorEpilogue e;
+        e.x = x; e.ldx = ldx;
+        e.d = tda ? ctx->diag_tda.as<double>() : ctx->diag_full.as<double>();
+        e.s = tda ? nullptr : ctx->sqrtr.as<double>();
+        e.beta = tda ? 2.0 : 4.0;
+        e.alpha = alpha;
+        e.v0 = v0; e.ldv0 = ldv0; e.gamma = gamma;
+        e.acc = acc; e.ldacc = ldacc;
+        e.out = out; e.ldo = ldo;
+        return e;
+    }
+};
+
+int gemm_f64(hipStream_t s, int m, int n, int k, const double* a, long lda, const double* b,
+             long ldb, double* c, long ldc);
+int k1_slots(int nv);
+int operator_f64(ngw_context* ctx, const OperatorArgs& a);
+int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int nvec, int tda,
+                 double* d_p, long ldp_out);
+
+// Dispatch on ctx->precision.
+int operator_apply(ngw_context* ctx, const OperatorArgs& a);
+int lowp_prepare(ngw_context* ctx);
+int operator_lowp(ngw_context* ctx, const OperatorArgs& a);
+
+// Lanczos pieces (lanczos.hip).
+int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double* d_h1, long ld,
+                int nvec, int niter, const double* poly, int npoly, int tda, double* d_a,
+                double* d_b);
+int scale_columns(hipStream_t s, const double* in, double* out, long ld, long n, int nvec,
+                  const double* d_norm);
+int column_norms(hipStream_t s, const double* v, long ld, long n, int nvec, double* d_norm2,
+                 double* d_norm);
+int continued_fraction(hipStream_t s, const double* d_a, const double* d_b, int niter, int nvec,
+                       const double* d_zz, int nz, double* d_spec);
+int zz_grid(hipStream_t s, const double* d_e1, const int* d_mband, const int* d_nband, int nval,
+            int nvec, int nen, double deltae, double ecuts, double* d_zz);
+int sigma_assemble(hipStream_t s, const double* d_spec, const double* d_zz, const double* d_pn2,
+                   int nen, int nqp, int nband, int nval, int tda, double* d_sigma);
+int qp_newton(hipStream_t s, int nqp, int nen, double deltae_ev, const double* d_in,
+              double* d_out, double* d_scratch);
+
+}  // namespace ngw

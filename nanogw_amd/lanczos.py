@@ -1,0 +1,194 @@
+"""Host-side mirror of NanoGW's Lanczos GW interfaces on top of libngw.
+
+Names, argument meaning and order follow the Fortran routines each method
+replaces; arrays are numpy, Fortran (column-major) order, energies in Ry.
+
+    ctx = IsdfContext(device=0)
+    ctx.set_isdf(psi_intp, mmtrx, e1, nval)           # type(ISDF) fields
+    hvec = ctx.matvec_isdf_BLOCK(vec)                  # apply_matvec_BLOCK.F90:25
+    hvec = ctx.polynomial_matvec_isdf_BLOCK(vec, polynomial)      # :210
+    spectra, a, b = ctx.lanczos_spectra_isdf_BLOCK(v0, v0_norm, zz, niter, polynomial)
+    sigma_c = ctx.calculate_sigma_lanczos(qp, nen, deltae, ecuts, niter, polynomial)
+    qp = ctx.qp_newton(...)                            # printsigma.F90:97-128
+
+Errors raise ``NgwError`` (NanoGW calls die()).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+PREC = {"f64": 0, "f32": 1, "bf16": 2}
+
+
+def _f64(a):
+    return np.require(a, np.float64, ["F_CONTIGUOUS", "ALIGNED"])
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def polyfit_sqrt(xmin, xmax, npoly, nx=20, chebyshev=False):
+    """shared/polyfit.f90:1 (chebyshev: polyfitcheb.f90:1).  Returns (beta low->high, fit_error)."""
+    beta = np.zeros(npoly + 1)
+    fe = np.zeros(1)
+    _lib.check(_lib.load().ngw_polyfit_sqrt(float(xmin), float(xmax), int(npoly), int(nx),
+                                            int(bool(chebyshev)), _p(beta), _p(fe)), "polyfit_sqrt")
+    return beta, float(fe[0])
+
+
+def lanczos_polynomial(e1, nval, npoly=12, nx=20, chebyshev=False):
+    """The driver's sqrt polynomial, highest order first
+    (calculate_sigma_lanczos_BLOCK.F90z:161-177)."""
+    e1 = np.asarray(e1, np.float64)
+    nb = len(e1)
+    emin = (e1[nval] - e1[nval - 1]) ** 2.0 / 1.25
+    emax = (e1[nb - 1] - e1[0]) ** 2.0 * 1.25
+    beta, _ = polyfit_sqrt(emin, emax, npoly, nx, chebyshev)
+    return beta[::-1].copy()
+
+
+class IsdfContext:
+    """Device-resident ISDF operator and Lanczos engine (one HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.ngw_create(C.byref(h), int(device)), "ngw_create")
+        self._h = h
+        self.nval = self.ncond = self.n_intp = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ngw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ------------------------------------------------------------ operator
+    def set_isdf(self, psi_intp, mmtrx, e1, nval):
+        psi_intp, mmtrx, e1 = _f64(psi_intp), _f64(mmtrx), _f64(e1)
+        n_intp, nband = psi_intp.shape
+        _lib.check(self._lib.ngw_set_isdf(self._h, n_intp, nband, int(nval), _p(psi_intp), n_intp,
+                                          _p(mmtrx), mmtrx.shape[0], _p(e1)), "ngw_set_isdf")
+        self.n_intp, self.nval, self.ncond = n_intp, int(nval), nband - int(nval)
+        self.nband = nband
+        self.e1 = e1.copy()
+
+    def set_precision(self, precision):
+        p = PREC[precision] if isinstance(precision, str) else int(precision)
+        _lib.check(self._lib.ngw_set_precision(self._h, p), "ngw_set_precision")
+
+    @property
+    def ncv(self):
+        return self.nval * self.ncond
+
+    def _vecs(self, vec):
+        v = _f64(np.asarray(vec, np.float64).reshape(self.ncv, -1, order="F"))
+        return v, v.shape[1]
+
+    def matvec_isdf_BLOCK(self, vec, tamm_d=False):
+        v, b = self._vecs(vec)
+        out = np.zeros_like(v, order="F")
+        _lib.check(self._lib.ngw_matvec_isdf_block(self._h, _p(v), _p(out), self.ncv, b, int(tamm_d)),
+                   "ngw_matvec_isdf_block")
+        return out
+
+    def polynomial_matvec_isdf_BLOCK(self, vec, polynomial, tamm_d=False):
+        v, b = self._vecs(vec)
+        poly = _f64(polynomial)
+        out = np.zeros_like(v, order="F")
+        _lib.check(self._lib.ngw_polynomial_matvec_isdf_block(self._h, _p(v), _p(out), self.ncv,
+                                                              _p(poly), len(poly) - 1, b,
+                                                              int(tamm_d)),
+                   "ngw_polynomial_matvec_isdf_block")
+        return out
+
+    def lanczos_spectra_isdf_BLOCK(self, v0, v0_norm, zz, niter, polynomial, tamm_d=False):
+        """Returns spectra (nz, blksz) complex and the a, b coefficients (niter, blksz)."""
+        v, b = self._vecs(v0)
+        vn = _f64(np.asarray(v0_norm, np.float64).reshape(b))
+        zz = np.require(np.asarray(zz, np.complex128).reshape(-1, b, order="F"), np.complex128,
+                        ["F_CONTIGUOUS"])
+        nz = zz.shape[0]
+        poly = _f64(polynomial)
+        spec = np.zeros((nz, b), np.complex128, order="F")
+        a = np.zeros((niter, b), order="F")
+        bb = np.zeros((niter, b), order="F")
+        _lib.check(self._lib.ngw_lanczos_spectra_isdf_block(
+            self._h, _p(v), _p(vn), self.ncv, zz.ctypes.data_as(C.POINTER(C.c_double)), nz,
+            int(niter), _p(poly), len(poly) - 1, spec.ctypes.data_as(C.POINTER(C.c_double)), b,
+            int(tamm_d), _p(a), _p(bb)), "ngw_lanczos_spectra_isdf_block")
+        return spec, a, bb
+
+    def calculate_sigma_lanczos(self, qp, nen, deltae, ecuts, niter, polynomial, tamm_d=False,
+                                max_cols=0):
+        """Sigma_c(z) (nen, nqp) complex in Ry for 0-based QP bands `qp`
+        (calculate_sigma_lanczos_BLOCK.F90z:201-444)."""
+        qp = np.ascontiguousarray(qp, np.int32)
+        poly = _f64(polynomial)
+        out = np.zeros((nen, len(qp)), np.complex128, order="F")
+        _lib.check(self._lib.ngw_sigma_lanczos(
+            self._h, len(qp), qp.ctypes.data_as(C.POINTER(C.c_int)), int(nen), float(deltae),
+            float(ecuts), int(niter), _p(poly), len(poly) - 1, int(tamm_d), int(max_cols),
+            out.ctypes.data_as(C.POINTER(C.c_double))), "ngw_sigma_lanczos")
+        return out
+
+    def qp_newton(self, sigma_c_ev, deltae_ev, edft_ev, en0_ev, sigx_ev, vxc_ev):
+        """printsigma.F90:97-128 on the device; all energies eV; sigma_c (nen, nqp)."""
+        sc = np.require(np.asarray(sigma_c_ev, np.complex128), np.complex128, ["F_CONTIGUOUS"])
+        nen, nqp = sc.shape
+        vals = [_f64(np.asarray(x, np.float64).reshape(nqp)) for x in (edft_ev, en0_ev, sigx_ev, vxc_ev)]
+        eqp, zfac, imsig = np.zeros(nqp), np.zeros(nqp), np.zeros(nqp)
+        s0 = np.zeros(nqp, np.complex128)
+        ierr = np.zeros(nqp, np.int32)
+        _lib.check(self._lib.ngw_qp_newton(
+            self._h, nqp, nen, float(deltae_ev), *[_p(v) for v in vals],
+            sc.ctypes.data_as(C.POINTER(C.c_double)), _p(eqp), _p(zfac),
+            s0.ctypes.data_as(C.POINTER(C.c_double)), _p(imsig),
+            ierr.ctypes.data_as(C.POINTER(C.c_int))), "ngw_qp_newton")
+        return {"eqp": eqp, "zfac": zfac, "sigma0": s0, "imsig": imsig, "ierr": ierr}
+
+    # ------------------------------------------------------------ V build
+    def isdf_coulomb_kernel(self, zeta, gidx, nfft, h, trunc, ntrans=1.0):
+        """M = (ntrans/h^3) zeta^T V zeta (isdf_parallel_sym_UltraLowMem.F90:686-875)."""
+        zeta = _f64(zeta)
+        ngrid, n_intp = zeta.shape
+        grid = np.ascontiguousarray(np.asarray(gidx, np.int32).reshape(ngrid, 3))
+        nf = np.ascontiguousarray(np.asarray(nfft, np.int32).reshape(3))
+        out = np.zeros((n_intp, n_intp), order="F")
+        _lib.check(self._lib.ngw_isdf_coulomb_kernel(
+            self._h, ngrid, n_intp, _p(zeta), zeta.shape[0], grid.ctypes.data_as(C.POINTER(C.c_int)),
+            nf.ctypes.data_as(C.POINTER(C.c_int)), float(h), float(trunc), float(ntrans), _p(out),
+            n_intp), "ngw_isdf_coulomb_kernel")
+        return out
+
+    # ------------------------------------------------------------ device pointers
+    def matvec_dev(self, d_vec: int, d_hvec: int, ld: int, blksz: int, tamm_d=False):
+        _lib.check(self._lib.ngw_matvec_isdf_block_dev(self._h, d_vec, d_hvec, int(ld), int(blksz),
+                                                       int(tamm_d)), "ngw_matvec_isdf_block_dev")
+
+    def gemm_f64_dev(self, m, n, k, d_a, lda, d_b, ldb, d_c, ldc):
+        _lib.check(self._lib.ngw_gemm_f64_dev(self._h, m, n, k, d_a, lda, d_b, ldb, d_c, ldc),
+                   "ngw_gemm_f64_dev")
+
+    def synchronize(self):
+        _lib.check(self._lib.ngw_synchronize(self._h), "ngw_synchronize")

@@ -1,0 +1,714 @@
+/*
+ * ngw_oracle.c -- CPU FP64 restatement of NanoGW's ISDF-Lanczos GW hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the CPU
+ * baseline ("port").  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker; the product library
+ * (nanogw_amd/libngw.so) never links, loads or calls it.
+ *
+ * Every routine follows one reference routine and cites it.  Array layouts
+ * are Fortran column-major, as in the reference:
+ *   psi(nmu, nb)      orbitals at the ISDF points, bands 1..nv occupied,
+ *                     nv+1..nb empty (Psi_intp_loc(:, :, 1, 1))
+ *   mmat(nmu, nmu)    ISDF Coulomb kernel (Mmtrx_loc(:, :, 1,1,1,1,1))
+ *   pair icv = iv + nv*ic  (0-based, conduction-outer / valence-inner,
+ *                     shared/setup_g.F90z:154-168)
+ *
+ * Pinning status (what checks each function; see tests/test_oracle*.py):
+ *   orc_lanczos_spectra_block : H_aux_v0.txt known-answer test (niter = 4),
+ *                               exact resolvent at niter = dim, moments.
+ *   orc_matvec_block          : dense closed form H = R^2 + 4 S C^T M C S
+ *                               (TDA: R + 2 C^T M C) built independently in
+ *                               numpy; symmetry x^T H y = y^T H x.
+ *   orc_polynomial_matvec_block : Z p(Lambda) Z^T from numpy eigh.
+ *   orc_sigma_lanczos         : sum-over-states with lambda_s = p(Omega_s^2)
+ *                               at niter = ncv (Lanczos exact limit).
+ *   orc_spline_evaluate / orc_newtonr / orc_qp_solve : exactness on cubic
+ *                               tables away from the ends, one-step Newton on
+ *                               linear f(E), hand-worked spline cases.
+ *   orc_polyfit_sqrt          : numpy.linalg.lstsq on the same nodes for
+ *                               well-conditioned degrees; at npoly = 12 the
+ *                               LAPACK dgesvx coefficients themselves are
+ *                               parity unpinned (cond ~ 1e38), so the
+ *                               coefficients are passed as pinned inputs.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define RYD 13.60569193 /* shared/myconstants.F90:41 */
+
+/* ------------------------------------------------------------------ */
+/* Small column-major GEMM helpers (the reference calls BLAS dgemm).   */
+/* ------------------------------------------------------------------ */
+
+/* C(m,n) = alpha*A(m,k)*B(k,n) + beta*C, all column-major. */
+static void gemm_nn(int m, int n, int k, double alpha, const double *A, long lda,
+                    const double *B, long ldb, double beta, double *C, long ldc)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < n; ++j) {
+        for (int i0 = 0; i0 < m; i0 += 256) {
+            int i1 = i0 + 256 < m ? i0 + 256 : m;
+            double acc[256];
+            for (int i = i0; i < i1; ++i) acc[i - i0] = 0.0;
+            for (int p = 0; p < k; ++p) {
+                double bp = B[p + (long)j * ldb];
+                if (bp == 0.0) continue;
+                const double *a = A + (long)p * lda;
+                for (int i = i0; i < i1; ++i) acc[i - i0] += a[i] * bp;
+            }
+            for (int i = i0; i < i1; ++i) {
+                double c = beta == 0.0 ? 0.0 : beta * C[i + (long)j * ldc];
+                C[i + (long)j * ldc] = c + alpha * acc[i - i0];
+            }
+        }
+    }
+}
+
+/* C(m,n) = alpha*A(k,m)^T*B(k,n) + beta*C. */
+static void gemm_tn(int m, int n, int k, double alpha, const double *A, long lda,
+                    const double *B, long ldb, double beta, double *C, long ldc)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < m; ++i) {
+            const double *a = A + (long)i * lda;
+            const double *b = B + (long)j * ldb;
+            double s = 0.0;
+            for (int p = 0; p < k; ++p) s += a[p] * b[p];
+            double c = beta == 0.0 ? 0.0 : beta * C[i + (long)j * ldc];
+            C[i + (long)j * ldc] = c + alpha * s;
+        }
+    }
+}
+
+static double ddot(long n, const double *x, const double *y)
+{
+    double s = 0.0;
+    for (long i = 0; i < n; ++i) s += x[i] * y[i];
+    return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* Operator apply                                                      */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Explicit ISDF coefficient matrix Cmtrx(mu, icv) = psi(mu, ic) * psi(mu, iv),
+ * the Hadamard product the reference rebuilds inside every operator call
+ * (shared/apply_matvec_BLOCK.F90:99-107, shared/hadamard_prod_mat_sym.cpp:3-17).
+ */
+static void build_cmtrx(int nmu, int nv, int nc, const double *psi, long ldpsi, double *cm)
+{
+    long ncv = (long)nv * nc;
+#pragma omp parallel for schedule(static)
+    for (long icv = 0; icv < ncv; ++icv) {
+        int iv = (int)(icv % nv);
+        int ic = nv + (int)(icv / nv);
+        const double *pv = psi + (long)iv * ldpsi;
+        const double *pc = psi + (long)ic * ldpsi;
+        double *col = cm + icv * nmu;
+        for (int mu = 0; mu < nmu; ++mu) col[mu] = pc[mu] * pv[mu];
+    }
+}
+
+/*
+ * Casida operator on blksz vectors, shared/apply_matvec_BLOCK.F90:25-208:
+ *   full (tda = 0): Hvec = R^2 vec + 4 R^1/2 C^T M C R^1/2 vec  (:53-55, :196-205)
+ *   TDA  (tda = 1): Hvec = R vec + 2 C^T M C vec   (apply_matvec_lowcomsym.F90:111-127, :396-403)
+ * with R^1/2 = sqrtR.  `cm` is an optional prebuilt Cmtrx (nmu x ncv); when
+ * NULL it is rebuilt here as the reference does on every call.
+ */
+void orc_matvec_block(int nmu, int nv, int nc, const double *psi, long ldpsi,
+                      const double *mmat, long ldm, const double *sqrtR,
+                      const double *vec, double *hvec, long ldvec, int blksz, int tda,
+                      const double *cm_in)
+{
+    long ncv = (long)nv * nc;
+    double *cm = (double *)cm_in;
+    if (!cm) {
+        cm = (double *)malloc(sizeof(double) * ncv * nmu);
+        build_cmtrx(nmu, nv, nc, psi, ldpsi, cm);
+    }
+    double *hs = (double *)malloc(sizeof(double) * ncv * blksz);
+    double *tv = (double *)malloc(sizeof(double) * (long)nmu * blksz);
+    double *mc = (double *)malloc(sizeof(double) * (long)nmu * blksz);
+    double *cmc = (double *)malloc(sizeof(double) * ncv * blksz);
+    /* Hvec = sqrtR * vec  (:53-55); TDA keeps vec (lowcomsym :111-116) */
+    for (int b = 0; b < blksz; ++b)
+        for (long i = 0; i < ncv; ++i)
+            hs[i + b * ncv] = tda ? vec[i + b * ldvec] : sqrtR[i] * vec[i + b * ldvec];
+    /* tmp_vec = Cmtrx @ Hvec  (:110-112) */
+    gemm_nn(nmu, blksz, (int)ncv, 1.0, cm, nmu, hs, ncv, 0.0, tv, nmu);
+    /* MC_vec = Mmtrx @ tmp_vec  (:139-142) */
+    gemm_nn(nmu, blksz, nmu, 1.0, mmat, ldm, tv, nmu, 0.0, mc, nmu);
+    /* CMC_vec = Cmtrx^T @ MC_vec  (:181-183) */
+    gemm_tn((int)ncv, blksz, nmu, 1.0, cm, nmu, mc, nmu, 0.0, cmc, ncv);
+    for (int b = 0; b < blksz; ++b) {
+        for (long i = 0; i < ncv; ++i) {
+            double s = sqrtR[i];
+            double h = hs[i + b * ncv];
+            double out;
+            if (tda) {
+                out = s * s * h;                  /* lowcomsym :399 */
+                out = out + 2.0 * cmc[i + b * ncv]; /* :401 */
+            } else {
+                out = s * s * s * h;                    /* BLOCK :198 */
+                out = out + 4.0 * s * cmc[i + b * ncv]; /* :202-203 */
+            }
+            hvec[i + b * ldvec] = out;
+        }
+    }
+    free(hs); free(tv); free(mc); free(cmc);
+    if (!cm_in) free(cm);
+}
+
+/*
+ * Horner evaluation of p(H) vec, coefficients high -> low
+ * (polynomial(1) = a_n), shared/apply_matvec_BLOCK.F90:210-268.
+ */
+void orc_polynomial_matvec_block(int nmu, int nv, int nc, const double *psi, long ldpsi,
+                                 const double *mmat, long ldm, const double *sqrtR,
+                                 const double *vec, double *hvec, int blksz,
+                                 const double *poly, int npoly, int tda, const double *cm)
+{
+    long ncv = (long)nv * nc;
+    double *hold = (double *)malloc(sizeof(double) * ncv * blksz);
+    for (int k = 1; k <= npoly; ++k) {
+        if (k == 1) {
+            orc_matvec_block(nmu, nv, nc, psi, ldpsi, mmat, ldm, sqrtR, vec, hvec, ncv,
+                             blksz, tda, cm);
+            for (long i = 0; i < ncv * blksz; ++i)
+                hvec[i] = poly[k - 1] * hvec[i] + poly[k] * vec[i]; /* :255 */
+        } else {
+            orc_matvec_block(nmu, nv, nc, psi, ldpsi, mmat, ldm, sqrtR, hold, hvec, ncv,
+                             blksz, tda, cm);
+            for (long i = 0; i < ncv * blksz; ++i)
+                hvec[i] = hvec[i] + poly[k] * vec[i]; /* :260 */
+        }
+        memcpy(hold, hvec, sizeof(double) * ncv * blksz); /* :264 */
+    }
+    free(hold);
+}
+
+/* ------------------------------------------------------------------ */
+/* Lanczos recurrence + continued fraction                              */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Continued fraction S(z) = <v|(z - T)^-1|v> from the Lanczos coefficients,
+ * shared/lanczos_spectra_BLOCK.F90:117-130 (numerator 1 for j = 1, b_j^2
+ * otherwise, recursed from j = niter down to 1).
+ */
+void orc_continued_fraction(int niter, const double *a, const double *b, int nz,
+                            const double complex *zz, double complex *spectra)
+{
+    for (int iz = 0; iz < nz; ++iz) {
+        double complex s = 0.0;
+        for (int j = niter; j >= 1; --j) {
+            double complex en = b[j - 1] * b[j - 1];
+            if (j == 1) en = 1.0;
+            double complex de = zz[iz] - a[j - 1] - s;
+            s = en / de;
+        }
+        spectra[iz] = s;
+    }
+}
+
+/*
+ * Generic Lanczos driver over a caller-supplied operator, following
+ * shared/lanczos_spectra_BLOCK.F90:7-134 (b independent three-term
+ * recurrences, no reorthogonalisation, no breakdown exit :80).
+ * apply(ctx, in, out, ncv, blksz) must compute out = p(H) in.
+ * a_out/b_out (niter x blksz) receive the coefficients when non-NULL.
+ */
+typedef void (*orc_apply_fn)(void *ctx, const double *in, double *out, long ncv, int blksz);
+
+static void lanczos_block_generic(orc_apply_fn apply, void *ctx, const double *v0,
+                                  const double *v0_norm, long ncv, const double complex *zz,
+                                  int nz, int niter, double complex *spectra, int blksz,
+                                  double *a_out, double *b_out)
+{
+    double *vec = (double *)malloc(sizeof(double) * ncv * blksz);
+    double *wvec = (double *)malloc(sizeof(double) * ncv * blksz);
+    double *tmp = (double *)malloc(sizeof(double) * ncv * blksz);
+    double *a = (double *)calloc((size_t)niter * blksz, sizeof(double));
+    double *b = (double *)calloc((size_t)niter * blksz, sizeof(double));
+    for (int ii = 0; ii < blksz; ++ii)
+        for (long i = 0; i < ncv; ++i) vec[i + ii * ncv] = v0[i + ii * ncv] / v0_norm[ii]; /* :40-42 */
+    for (int ii = 0; ii < blksz; ++ii) b[0 + ii * niter] = 1.0; /* :47 */
+    apply(ctx, vec, wvec, ncv, blksz);                        /* :54 */
+    for (int ii = 0; ii < blksz; ++ii) {                       /* :63-77 */
+        double *w = wvec + ii * ncv, *q = vec + ii * ncv;
+        double al = ddot(ncv, w, q);
+        a[0 + ii * niter] = al;
+        for (long i = 0; i < ncv; ++i) w[i] += -al * q[i];
+        if (niter >= 2) b[1 + ii * niter] = sqrt(ddot(ncv, w, w));
+    }
+    for (int k = 2; k <= niter; ++k) { /* :79-108 */
+        for (int ii = 0; ii < blksz; ++ii) {
+            double bk = b[(k - 1) + ii * niter];
+            double *w = wvec + ii * ncv, *q = vec + ii * ncv;
+            for (long i = 0; i < ncv; ++i) {
+                w[i] = w[i] / bk;
+                q[i] = -q[i] * bk;
+            }
+        }
+        double *t = vec; vec = wvec; wvec = t; /* swap :86-88 */
+        apply(ctx, vec, tmp, ncv, blksz);      /* :91-93 */
+        for (long i = 0; i < ncv * blksz; ++i) wvec[i] = wvec[i] + tmp[i]; /* :95 */
+        for (int ii = 0; ii < blksz; ++ii) {
+            double *w = wvec + ii * ncv, *q = vec + ii * ncv;
+            double al = ddot(ncv, w, q);
+            a[(k - 1) + ii * niter] = al;
+            if (k + 1 <= niter) {
+                for (long i = 0; i < ncv; ++i) w[i] += -al * q[i];
+                b[k + ii * niter] = sqrt(ddot(ncv, w, w));
+            }
+        }
+    }
+    for (int ii = 0; ii < blksz; ++ii)
+        orc_continued_fraction(niter, a + ii * niter, b + ii * niter, nz, zz + (long)ii * nz,
+                               spectra + (long)ii * nz);
+    if (a_out) memcpy(a_out, a, sizeof(double) * niter * blksz);
+    if (b_out) memcpy(b_out, b, sizeof(double) * niter * blksz);
+    free(vec); free(wvec); free(tmp); free(a); free(b);
+}
+
+/* Dense operator: out = H in, H given explicitly (n x n, column-major). */
+struct dense_ctx { const double *h; long n; };
+static void dense_apply(void *ctx, const double *in, double *out, long ncv, int blksz)
+{
+    struct dense_ctx *d = (struct dense_ctx *)ctx;
+    gemm_nn((int)ncv, blksz, (int)ncv, 1.0, d->h, d->n, in, ncv, 0.0, out, ncv);
+}
+
+/* Lanczos on an explicit symmetric matrix, as shared/test_lanczos does with
+ * H_aux (shared/lanczos_spectra.f90:7-101 with the BLOCK variant's loop). */
+void orc_lanczos_dense(const double *h, long n, const double *v0, double v0_norm,
+                       const double complex *zz, int nz, int niter, double complex *spectra,
+                       double *a_out, double *b_out)
+{
+    struct dense_ctx d = {h, n};
+    lanczos_block_generic(dense_apply, &d, v0, &v0_norm, n, zz, nz, niter, spectra, 1,
+                          a_out, b_out);
+}
+
+struct isdf_ctx {
+    int nmu, nv, nc; const double *psi; long ldpsi; const double *mmat; long ldm;
+    const double *sqrtR; const double *poly; int npoly; int tda; const double *cm;
+};
+static void isdf_apply(void *ctx, const double *in, double *out, long ncv, int blksz)
+{
+    struct isdf_ctx *c = (struct isdf_ctx *)ctx;
+    (void)ncv;
+    orc_polynomial_matvec_block(c->nmu, c->nv, c->nc, c->psi, c->ldpsi, c->mmat, c->ldm,
+                                c->sqrtR, in, out, blksz, c->poly, c->npoly, c->tda, c->cm);
+}
+
+/* shared/lanczos_spectra_BLOCK.F90:7-134 on the ISDF operator. */
+void orc_lanczos_spectra_block(int nmu, int nv, int nc, const double *psi, long ldpsi,
+                               const double *mmat, long ldm, const double *sqrtR,
+                               const double *v0, const double *v0_norm,
+                               const double complex *zz, int nz, int niter,
+                               const double *poly, int npoly, double complex *spectra,
+                               int blksz, int tda, double *a_out, double *b_out)
+{
+    long ncv = (long)nv * nc;
+    double *cm = (double *)malloc(sizeof(double) * ncv * nmu);
+    build_cmtrx(nmu, nv, nc, psi, ldpsi, cm);
+    struct isdf_ctx c = {nmu, nv, nc, psi, ldpsi, mmat, ldm, sqrtR, poly, npoly, tda, cm};
+    lanczos_block_generic(isdf_apply, &c, v0, v0_norm, ncv, zz, nz, niter, spectra, blksz,
+                          a_out, b_out);
+    free(cm);
+}
+
+/* ------------------------------------------------------------------ */
+/* Sigma_c driver                                                       */
+/* ------------------------------------------------------------------ */
+
+/*
+ * P vectors for QP state m (0-based band index) and bands n0..n0+blksz-1:
+ *   tmp_Cvec(mu,kk) = psi(mu,m) psi(mu,n)            (BLOCK driver :286-291)
+ *   tmp_CvecM = Mmtrx @ tmp_Cvec                     (:304-306)
+ *   Pvec = Cmtrx^T tmp_CvecM                          (:316-345)
+ *   Pvec *= sqrtR (full only), pnorm2 = |Pvec|^2      (:349-358; lowcomsym :700-710)
+ */
+void orc_pvectors(int nmu, int nv, int nc, const double *psi, long ldpsi, const double *mmat,
+                  long ldm, const double *sqrtR, int m, int n0, int blksz, int tda,
+                  const double *cm, double *pvec, double *pnorm2)
+{
+    long ncv = (long)nv * nc;
+    double *cv = (double *)malloc(sizeof(double) * (long)nmu * blksz);
+    double *cvm = (double *)malloc(sizeof(double) * (long)nmu * blksz);
+    for (int kk = 0; kk < blksz; ++kk)
+        for (int mu = 0; mu < nmu; ++mu)
+            cv[mu + (long)kk * nmu] = psi[mu + (long)m * ldpsi] * psi[mu + (long)(n0 + kk) * ldpsi];
+    gemm_nn(nmu, blksz, nmu, 1.0, mmat, ldm, cv, nmu, 0.0, cvm, nmu);
+    gemm_tn((int)ncv, blksz, nmu, 1.0, cm, nmu, cvm, nmu, 0.0, pvec, ncv);
+    for (int kk = 0; kk < blksz; ++kk) {
+        double *p = pvec + (long)kk * ncv;
+        if (!tda)
+            for (long i = 0; i < ncv; ++i) p[i] = p[i] * sqrtR[i];
+        pnorm2[kk] = ddot(ncv, p, p);
+    }
+    free(cv); free(cvm);
+}
+
+/*
+ * Sigma_c(omega) on the nen-point grid for each requested QP state,
+ * SIGMA/calculate_sigma_lanczos_BLOCK.F90z:201-444, TDA branch from
+ * SIGMA/calculate_sigma_lanczos_lowcomsym.F90z:712-745.
+ *   e1(nb)    orbital energies (Ry);  qp(nqp) 0-based band of each QP state
+ *   deltae    energy_range (Ry);  ecuts  dynamic_energy_resolution (Ry)
+ *   sigma(nen, nqp) complex, Ry.
+ */
+void orc_sigma_lanczos(int nmu, int nv, int nc, const double *psi, long ldpsi,
+                       const double *mmat, long ldm, const double *e1, int nqp, const int *qp,
+                       int nen, double deltae, double ecuts, int niter, const double *poly,
+                       int npoly, int jnblock, int tda, double complex *sigma)
+{
+    int nb = nv + nc;
+    long ncv = (long)nv * nc;
+    double *sqrtR = (double *)malloc(sizeof(double) * ncv);
+    for (long icv = 0; icv < ncv; ++icv) {
+        int iv = (int)(icv % nv), ic = nv + (int)(icv / nv);
+        sqrtR[icv] = pow(e1[ic] - e1[iv], 0.5); /* :201-208 */
+    }
+    double *cm = (double *)malloc(sizeof(double) * ncv * nmu);
+    build_cmtrx(nmu, nv, nc, psi, ldpsi, cm);
+    /* band blocks, occupied first (:210-245) */
+    int nblks = 0;
+    int (*blks)[3] = malloc(sizeof(int[3]) * (nb / (jnblock > 0 ? jnblock : 1) + 4));
+    for (int s = 0; s < nv; s += jnblock) {
+        blks[nblks][0] = s; blks[nblks][1] = (s + jnblock < nv ? s + jnblock : nv) - 1; blks[nblks][2] = 1; ++nblks;
+    }
+    for (int s = nv; s < nb; s += jnblock) {
+        blks[nblks][0] = s; blks[nblks][1] = (s + jnblock < nb ? s + jnblock : nb) - 1; blks[nblks][2] = 0; ++nblks;
+    }
+    double *pvec = (double *)malloc(sizeof(double) * ncv * jnblock);
+    double *pnorm2 = (double *)malloc(sizeof(double) * jnblock);
+    double *pnorm = (double *)malloc(sizeof(double) * jnblock);
+    double complex *ezz = (double complex *)malloc(sizeof(double complex) * (nen + 1) * jnblock);
+    double complex *sct = (double complex *)malloc(sizeof(double complex) * (nen + 1) * jnblock);
+    double de = deltae / (nen - 1); /* :254 */
+    struct isdf_ctx c = {nmu, nv, nc, psi, ldpsi, mmat, ldm, sqrtR, poly, npoly, tda, cm};
+    for (int isig = 0; isig < nqp; ++isig) {
+        int m1 = qp[isig];
+        double complex *sc = sigma + (long)isig * nen;
+        for (int i = 0; i < nen; ++i) sc[i] = 0.0;
+        double en0 = e1[m1] - deltae * 0.5; /* :260 */
+        for (int ib = 0; ib < nblks; ++ib) {
+            int jn = blks[ib][0], blksz = blks[ib][1] - blks[ib][0] + 1;
+            int occ = blks[ib][2];
+            orc_pvectors(nmu, nv, nc, psi, ldpsi, mmat, ldm, sqrtR, m1, jn, blksz, tda, cm, pvec,
+                         pnorm2);
+            for (int kk = 0; kk < blksz; ++kk) pnorm[kk] = sqrt(pnorm2[kk]);
+            double eta = occ ? 1.0 : -1.0;
+            for (int kk = 0; kk < blksz; ++kk) {
+                for (int i = 0; i < nen; ++i) {
+                    double zz = de * i + en0; /* :261-263 */
+                    /* occupied: -(z - e_n) + i ecuts (:366-368); empty: (z - e_n) + i ecuts (:393-395) */
+                    ezz[i + (long)kk * (nen + 1)] = eta * (e1[jn + kk] - zz) + I * ecuts;
+                }
+                ezz[nen + (long)kk * (nen + 1)] = 0.0; /* :371 */
+            }
+            lanczos_block_generic(isdf_apply, &c, pvec, pnorm, ncv, ezz, nen + 1, niter, sct,
+                                  blksz, NULL, NULL);
+            for (int kk = 0; kk < blksz; ++kk) {
+                double complex *st = sct + (long)kk * (nen + 1);
+                double complex *ez = ezz + (long)kk * (nen + 1);
+                for (int i = 0; i < nen; ++i) {
+                    double complex inc;
+                    if (tda) /* lowcomsym :738-740; 2.0**0.5 is default (single) real */
+                        inc = -st[i] * eta * (double)sqrtf(2.0f) * pnorm2[kk];
+                    else if (occ) /* BLOCK :382-383 */
+                        inc = 2.0 * pnorm2[kk] / (-ez[i]) * (-st[nen] + st[i]);
+                    else /* BLOCK :408-409 */
+                        inc = 2.0 * pnorm2[kk] / ez[i] * (-st[nen] + st[i]);
+                    sc[i] = sc[i] + inc;
+                }
+            }
+        }
+    }
+    free(sqrtR); free(cm); free(blks); free(pvec); free(pnorm2); free(pnorm); free(ezz); free(sct);
+}
+
+/* ------------------------------------------------------------------ */
+/* QP energy: spline + Newton                                           */
+/* ------------------------------------------------------------------ */
+
+/* SIGMA/spline.f90:20-77 on one real component; y2 output (nn). */
+static void spline_real(const double *xx, const double *yy, int nn, double yp1, double ypn,
+                        double *y2)
+{
+    const double HUGEV = 1.0e10; /* declared as default real (:33), exactly 1e10 */
+    double *uu = (double *)malloc(sizeof(double) * nn);
+    if (fabs(yp1) > HUGEV) {
+        y2[0] = 0.0; uu[0] = 0.0;
+    } else {
+        y2[0] = -0.5;
+        uu[0] = (3.0 / (xx[1] - xx[0])) * ((yy[1] - yy[0]) / (xx[1] - xx[0]) - yp1);
+    }
+    for (int i = 1; i < nn - 1; ++i) {
+        double sig = (xx[i] - xx[i - 1]) / (xx[i + 1] - xx[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        uu[i] = (6.0 * ((yy[i + 1] - yy[i]) / (xx[i + 1] - xx[i]) - (yy[i] - yy[i - 1]) / (xx[i] - xx[i - 1])) /
+                     (xx[i + 1] - xx[i - 1]) - sig * uu[i - 1]) / p;
+    }
+    double qn, un;
+    if (ypn > HUGEV) { /* no abs() here in the reference (:172) */
+        qn = 0.0; un = 0.0;
+    } else {
+        qn = 0.5;
+        un = (3.0 / (xx[nn - 1] - xx[nn - 2])) * (ypn - (yy[nn - 1] - yy[nn - 2]) / (xx[nn - 1] - xx[nn - 2]));
+    }
+    y2[nn - 1] = (un - qn * uu[nn - 2]) / (qn * y2[nn - 2] + 1.0);
+    for (int k = nn - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + uu[k];
+    free(uu);
+}
+
+/* SIGMA/splint.f90:12-53 (bisection + cubic evaluation). */
+static double complex splint_c(const double *xx, const double complex *yy, const double complex *y2a,
+                               int nn, double x0)
+{
+    int klo = 1, khi = nn;
+    while (khi - klo > 1) {
+        int kk = (khi + klo) / 2;
+        if (xx[kk - 1] > x0) khi = kk; else klo = kk;
+    }
+    double hh = xx[khi - 1] - xx[klo - 1];
+    double aa = (xx[khi - 1] - x0) / hh;
+    double bb = (x0 - xx[klo - 1]) / hh;
+    return aa * yy[klo - 1] + bb * yy[khi - 1] +
+           ((aa * aa * aa - aa) * y2a[klo - 1] + (bb * bb * bb - bb) * y2a[khi - 1]) * (hh * hh) / 6.0;
+}
+
+/*
+ * SIGMA/spline_evaluate.f90:13-51: spline through points 2..n-1 with end
+ * slopes from the first and last pairs; df = Re central difference, +-0.01.
+ */
+void orc_spline_evaluate(int ntable, const double *xtable, const double complex *ftable, double xx,
+                         double complex *fx, double *df)
+{
+    int nn = ntable - 2;
+    double complex yp1 = (ftable[1] - ftable[0]) / (xtable[1] - xtable[0]);
+    double complex ypn = (ftable[ntable - 1] - ftable[ntable - 2]) / (xtable[ntable - 1] - xtable[ntable - 2]);
+    double *yr = (double *)malloc(sizeof(double) * nn), *yi = (double *)malloc(sizeof(double) * nn);
+    double *y2r = (double *)malloc(sizeof(double) * nn), *y2i = (double *)malloc(sizeof(double) * nn);
+    double complex *y2 = (double complex *)malloc(sizeof(double complex) * nn);
+    for (int i = 0; i < nn; ++i) { yr[i] = creal(ftable[i + 1]); yi[i] = cimag(ftable[i + 1]); }
+    spline_real(xtable + 1, yr, nn, creal(yp1), creal(ypn), y2r);
+    spline_real(xtable + 1, yi, nn, cimag(yp1), cimag(ypn), y2i);
+    for (int i = 0; i < nn; ++i) y2[i] = y2r[i] + I * y2i[i];
+    const double DELTAX = 1.0e-2;
+    *fx = splint_c(xtable + 1, ftable + 1, y2, nn, xx);
+    double complex ym = splint_c(xtable + 1, ftable + 1, y2, nn, xx - DELTAX);
+    double complex yp = splint_c(xtable + 1, ftable + 1, y2, nn, xx + DELTAX);
+    *df = creal(yp - ym) / DELTAX * 0.5;
+    free(yr); free(yi); free(y2r); free(y2i); free(y2);
+}
+
+/* SIGMA/newtonr.f90:18-58; returns ierr (0 on convergence, MAX_IT+1 otherwise). */
+int orc_newtonr(int ntable, const double *xtable, const double complex *ftable, double *x_zero,
+                double xacc, double complex *fx)
+{
+    double df;
+    int jj;
+    for (jj = 1; jj <= 20; ++jj) {
+        orc_spline_evaluate(ntable, xtable, ftable, *x_zero, fx, &df);
+        double dx = creal(*fx) / df;
+        *x_zero = *x_zero - dx;
+        if (fabs(dx) < xacc) {
+            orc_spline_evaluate(ntable, xtable, ftable, *x_zero, fx, &df);
+            return 0;
+        }
+    }
+    return jj;
+}
+
+/*
+ * QP solve for one state, SIGMA/printsigma.F90:97-128 with Sigma_g = 0.
+ * Inputs in eV (the driver multiplies by ryd before printsigma, :474-501):
+ *   edft = e0(m)*ryd, en0 = e1(m)*ryd, sigx, vxc (real parts), sigc(nen)
+ *   deltae_ev = deltae*ryd.
+ * Outputs: eqp, zfac, sigma0 (= Sigma(E_qp) + ... see :121-123), imsig.
+ */
+int orc_qp_solve(int nen, double deltae_ev, double edft, double en0, double sigx, double vxc,
+                 const double complex *sigc, double *eqp, double *zfac, double complex *sigma0,
+                 double *imsig)
+{
+    double *xt = (double *)malloc(sizeof(double) * nen);
+    double complex *ft = (double complex *)malloc(sizeof(double complex) * nen);
+    double complex *it = (double complex *)malloc(sizeof(double complex) * nen);
+    for (int ii = 0; ii < nen; ++ii) {
+        xt[ii] = en0 - deltae_ev * 0.5 + deltae_ev / (double)(nen - 1) * (double)ii; /* :101-102 */
+        /* scdiag(:, isig) holds Re Sigma_c and scdiag(:, ndiag+isig) Im Sigma_c as
+           separate real tables (calculate_sigma_lanczos_BLOCK.F90z:441-444) */
+        ft[ii] = sigx + creal(sigc[ii]) - vxc + edft - xt[ii];                        /* :103-104 */
+        it[ii] = cimag(sigc[ii]);                                                      /* :105-106 */
+    }
+    double complex ztmp, s0;
+    double zf, xtmp;
+    double e = en0; /* :110 */
+    int ierr = orc_newtonr(nen, xt, ft, &e, 0.01, &ztmp); /* EN_ACC = 0.01 (:60) */
+    orc_spline_evaluate(nen, xt, ft, e, &s0, &zf);        /* :121 */
+    orc_spline_evaluate(nen, xt, ft, edft, &s0, &xtmp);   /* :122 */
+    s0 = s0 + vxc;                                        /* :123 */
+    zf = -1.0 / zf;                                       /* :124 */
+    double complex si;
+    orc_spline_evaluate(nen, xt, it, e, &si, &xtmp);      /* :128 */
+    *eqp = e; *zfac = zf; *sigma0 = s0; *imsig = creal(si);
+    free(xt); free(ft); free(it);
+    return ierr;
+}
+
+/* ------------------------------------------------------------------ */
+/* sqrt polynomial fit                                                  */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Solve A x = b (n x n) the way LAPACK dgesvx(fact='E') does it: dgeequ
+ * row/column equilibration applied only when dlaqge's thresholds call for it,
+ * LU with partial pivoting, dgerfs-style iterative refinement (<= 5 steps),
+ * then unscaling (shared/linear_solver.f90:206-215).
+ */
+static void solve_equilibrated(int n, const double *Ain, const double *bin, double *x)
+{
+    double *A = (double *)malloc(sizeof(double) * n * n), *LU = (double *)malloc(sizeof(double) * n * n);
+    double *r = (double *)malloc(sizeof(double) * n), *c = (double *)malloc(sizeof(double) * n);
+    double *b = (double *)malloc(sizeof(double) * n), *res = (double *)malloc(sizeof(double) * n);
+    double *dx = (double *)malloc(sizeof(double) * n);
+    int *piv = (int *)malloc(sizeof(int) * n);
+    memcpy(A, Ain, sizeof(double) * n * n);
+    /* dgeequ */
+    double rmin = 1e300, rmax = 0.0, amax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double m = 0.0;
+        for (int j = 0; j < n; ++j) m = fmax(m, fabs(A[i + j * n]));
+        r[i] = m; amax = fmax(amax, m);
+        rmin = fmin(rmin, m); rmax = fmax(rmax, m);
+    }
+    for (int i = 0; i < n; ++i) r[i] = 1.0 / r[i];
+    double rowcnd = rmin / rmax;
+    double cmin = 1e300, cmax = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double m = 0.0;
+        for (int i = 0; i < n; ++i) m = fmax(m, fabs(A[i + j * n]) * r[i]);
+        c[j] = m; cmin = fmin(cmin, m); cmax = fmax(cmax, m);
+    }
+    for (int j = 0; j < n; ++j) c[j] = 1.0 / c[j];
+    double colcnd = cmin / cmax;
+    /* dlaqge: THRESH = 0.1, small = safmin/eps, large = 1/small */
+    const double small = 2.2250738585072014e-308 / 1.1102230246251565e-16, large = 1.0 / small;
+    int rowscale = !(rowcnd >= 0.1 && amax >= small && amax <= large);
+    int colscale = !(colcnd >= 0.1);
+    if (!rowscale) for (int i = 0; i < n; ++i) r[i] = 1.0;
+    if (!colscale) for (int j = 0; j < n; ++j) c[j] = 1.0;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) A[i + j * n] *= r[i] * c[j];
+    for (int i = 0; i < n; ++i) b[i] = bin[i] * r[i];
+    /* dgetrf (right-looking, partial pivoting) */
+    memcpy(LU, A, sizeof(double) * n * n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i) if (fabs(LU[i + k * n]) > fabs(LU[p + k * n])) p = i;
+        piv[k] = p;
+        if (p != k) for (int j = 0; j < n; ++j) { double t = LU[k + j * n]; LU[k + j * n] = LU[p + j * n]; LU[p + j * n] = t; }
+        double d = LU[k + k * n];
+        for (int i = k + 1; i < n; ++i) LU[i + k * n] /= d;
+        for (int j = k + 1; j < n; ++j)
+            for (int i = k + 1; i < n; ++i) LU[i + j * n] -= LU[i + k * n] * LU[k + j * n];
+    }
+#define LU_SOLVE(vec)                                                           \
+    do {                                                                        \
+        for (int k = 0; k < n; ++k) if (piv[k] != k) { double t = vec[k]; vec[k] = vec[piv[k]]; vec[piv[k]] = t; } \
+        for (int j = 0; j < n; ++j) for (int i = j + 1; i < n; ++i) vec[i] -= LU[i + j * n] * vec[j]; \
+        for (int j = n - 1; j >= 0; --j) { vec[j] /= LU[j + j * n]; for (int i = 0; i < j; ++i) vec[i] -= LU[i + j * n] * vec[j]; } \
+    } while (0)
+    memcpy(x, b, sizeof(double) * n);
+    LU_SOLVE(x);
+    /* dgerfs: refine while the componentwise backward error shrinks */
+    const double eps = 1.1102230246251565e-16;
+    double lastberr = 1e300;
+    for (int it = 0; it < 5; ++it) {
+        double berr = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double s = b[i], den = fabs(b[i]);
+            for (int j = 0; j < n; ++j) { s -= A[i + j * n] * x[j]; den += fabs(A[i + j * n] * x[j]); }
+            res[i] = s;
+            if (den > 0) berr = fmax(berr, fabs(s) / den);
+        }
+        if (!(berr > eps && 2.0 * berr <= lastberr)) break;
+        memcpy(dx, res, sizeof(double) * n);
+        LU_SOLVE(dx);
+        for (int i = 0; i < n; ++i) x[i] += dx[i];
+        lastberr = berr;
+    }
+#undef LU_SOLVE
+    for (int i = 0; i < n; ++i) x[i] *= c[i];
+    free(A); free(LU); free(r); free(c); free(b); free(res); free(dx); free(piv);
+}
+
+/*
+ * Least-squares monomial fit of sqrt(x) on [xmin, xmax] through nx nodes,
+ * equispaced (shared/polyfit.f90:1-90) or Chebyshev (cheb = 1,
+ * shared/polyfitcheb.f90:1-96); normal equations M = P^T P, y = P^T sqrt(x).
+ * beta(npoly+1) is low -> high (beta(1) = a_0); returns -1 on bad range.
+ */
+int orc_polyfit_sqrt(double xmin, double xmax, int npoly, int nx, int cheb, double *beta,
+                     double *fit_error)
+{
+    if (xmin >= xmax) { double t = xmin; xmin = xmax; xmax = t; }
+    if (xmin <= 0.0 || xmin <= 1e-4) return -1; /* die() in :27-34 */
+    int np = npoly + 1;
+    double *xx = (double *)malloc(sizeof(double) * nx), *yy = (double *)malloc(sizeof(double) * nx);
+    double *pxx = (double *)malloc(sizeof(double) * nx * np);
+    double *Mm = (double *)malloc(sizeof(double) * np * np), *y = (double *)malloc(sizeof(double) * np);
+    const double pi = 3.1415926535897932384626433832795;
+    double emid = 0.5 * (xmin + xmax), scal = 0.5 * (xmax - xmin);
+    double dx = cheb ? pi / (nx - 1) : (xmax - xmin) / (nx - 1);
+    for (int ix = 0; ix < nx; ++ix) {
+        xx[ix] = cheb ? emid - scal * cos(dx * ix) : xmin + dx * ix;
+        yy[ix] = sqrt(xx[ix]);
+        pxx[ix] = 1.0;
+        for (int ip = 1; ip < np; ++ip) pxx[ix + ip * nx] = pow(xx[ix], (double)ip);
+    }
+    for (int i = 0; i < np; ++i) {
+        for (int j = 0; j < np; ++j) {
+            double s = 0.0;
+            for (int ix = 0; ix < nx; ++ix) s += pxx[ix + i * nx] * pxx[ix + j * nx];
+            Mm[i + j * np] = s;
+        }
+        double s = 0.0;
+        for (int ix = 0; ix < nx; ++ix) s += pxx[ix + i * nx] * yy[ix];
+        y[i] = s;
+    }
+    solve_equilibrated(np, Mm, y, beta);
+    const int mx = 2000;
+    double fe = 0.0;
+    for (int ix = 0; ix < mx; ++ix) {
+        double tx = xmin + ix * (xmax - xmin) / (mx - 1), tp = 1.0, fy = 0.0;
+        for (int ip = 0; ip < np; ++ip) { fy += tp * beta[ip]; if (ip < np - 1) tp *= tx; }
+        fe += fabs((sqrt(tx) - fy) / sqrt(tx)) / (double)mx;
+    }
+    if (fit_error) *fit_error = fe;
+    free(xx); free(yy); free(pxx); free(Mm); free(y);
+    return 0;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,283 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
+import this module, and only to check or time the product; the product
+package ``nanogw_amd`` never imports it.
+
+The C restatement lives in ``ngw_oracle.c`` (each function cites the NanoGW
+routine it follows).  The FFT Poisson solve and the ISDF Coulomb-kernel build
+are restated here with numpy's pocketfft (``poisson``, ``isdf_kernel``)
+following shared/fft_module.F90z:122-177, 628-691 and
+shared/isdf_parallel_sym_UltraLowMem.F90:686-875.
+
+Pinning (tests/test_oracle_*.py):
+  * lanczos/continued fraction: H_aux_v0.txt known answer, exact resolvent,
+    moment identity;
+  * operator, Horner, Sigma_c: dense closed forms built independently with
+    numpy eigh, sum over states;
+  * poisson: analytic Coulomb energy of Gaussian charges (erf form);
+    isdf_kernel: symmetry, positive semi-definiteness, Nmu = Ngrid limit;
+  * spline/Newton: exactness on cubic tables, one-step Newton on linear f;
+  * polyfit: numpy.linalg.lstsq on well-conditioned cases.  The npoly = 12
+    LAPACK dgesvx coefficients are parity unpinned; they are pinned inputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+RYD = 13.60569193  # shared/myconstants.F90:41
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build() -> str:
+    """Compile ngw_oracle.c into libngw_oracle.so (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "libngw_oracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libngw_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_matvec_block.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
+                                       _dp, _dp, _dp, C.c_long, C.c_int, C.c_int, _dp]
+        L.orc_polynomial_matvec_block.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp,
+                                                  C.c_long, _dp, _dp, _dp, C.c_int, _dp, C.c_int,
+                                                  C.c_int, _dp]
+        L.orc_continued_fraction.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+        L.orc_lanczos_dense.argtypes = [_dp, C.c_long, _dp, C.c_double, _dp, C.c_int, C.c_int,
+                                        _dp, _dp, _dp]
+        L.orc_lanczos_spectra_block.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp,
+                                                C.c_long, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
+                                                _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp]
+        L.orc_pvectors.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long, _dp,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_sigma_lanczos.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
+                                        _dp, C.c_int, _ip, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_spline_evaluate.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
+        L.orc_newtonr.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_newtonr.restype = C.c_int
+        L.orc_qp_solve.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.orc_qp_solve.restype = C.c_int
+        L.orc_polyfit_sqrt.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.orc_polyfit_sqrt.restype = C.c_int
+        L.orc_num_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.require(a, np.float64, ["F_CONTIGUOUS", "ALIGNED"])
+
+
+def _c128(a):
+    return np.require(a, np.complex128, ["F_CONTIGUOUS", "ALIGNED"])
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def sqrt_r(e1, nv, nc):
+    """sqrtR(icv) = (e_c - e_v)^1/2, icv = iv + nv*ic (calculate_sigma_lanczos_BLOCK.F90z:201-208)."""
+    e1 = np.asarray(e1, np.float64)
+    return np.sqrt((e1[nv:nv + nc][:, None] - e1[:nv][None, :]).reshape(-1))
+
+
+def matvec_block(psi, mmat, sqrtR, vec, nv, nc, tda=False):
+    psi, mmat, sqrtR = _f64(psi), _f64(mmat), _f64(sqrtR)
+    vec = _f64(vec.reshape(nv * nc, -1, order="F"))
+    out = np.zeros_like(vec, order="F")
+    nmu = psi.shape[0]
+    lib().orc_matvec_block(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0], _p(sqrtR),
+                           _p(vec), _p(out), vec.shape[0], vec.shape[1], int(tda), None)
+    return out
+
+
+def polynomial_matvec_block(psi, mmat, sqrtR, vec, nv, nc, poly, tda=False):
+    psi, mmat, sqrtR, poly = _f64(psi), _f64(mmat), _f64(sqrtR), _f64(poly)
+    vec = _f64(vec.reshape(nv * nc, -1, order="F"))
+    out = np.zeros_like(vec, order="F")
+    nmu = psi.shape[0]
+    lib().orc_polynomial_matvec_block(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0],
+                                      _p(sqrtR), _p(vec), _p(out), vec.shape[1], _p(poly),
+                                      len(poly) - 1, int(tda), None)
+    return out
+
+
+def continued_fraction(a, b, zz):
+    a, b, zz = _f64(a), _f64(b), _c128(zz)
+    out = np.zeros(len(zz), np.complex128)
+    lib().orc_continued_fraction(len(a), _p(a), _p(b), len(zz), _p(zz), _p(out))
+    return out
+
+
+def lanczos_dense(h, v0, zz, niter):
+    h, v0, zz = _f64(h), _f64(v0), _c128(zz)
+    n = h.shape[0]
+    spec = np.zeros(len(zz), np.complex128)
+    a = np.zeros(niter)
+    b = np.zeros(niter)
+    lib().orc_lanczos_dense(_p(h), n, _p(v0), float(np.linalg.norm(v0)), _p(zz), len(zz), niter,
+                            _p(spec), _p(a), _p(b))
+    return spec, a, b
+
+
+def lanczos_spectra_block(psi, mmat, sqrtR, v0, v0_norm, zz, niter, poly, nv, nc, tda=False):
+    """zz: (nz, blksz) complex.  Returns spectra (nz, blksz), a, b (niter, blksz)."""
+    psi, mmat, sqrtR, poly = _f64(psi), _f64(mmat), _f64(sqrtR), _f64(poly)
+    v0 = _f64(v0.reshape(nv * nc, -1, order="F"))
+    blksz = v0.shape[1]
+    v0_norm = _f64(np.asarray(v0_norm, np.float64).reshape(blksz))
+    zz = _c128(np.asarray(zz).reshape(-1, blksz, order="F"))
+    nz = zz.shape[0]
+    spec = np.zeros((nz, blksz), np.complex128, order="F")
+    a = np.zeros((niter, blksz), order="F")
+    b = np.zeros((niter, blksz), order="F")
+    nmu = psi.shape[0]
+    lib().orc_lanczos_spectra_block(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0],
+                                    _p(sqrtR), _p(v0), _p(v0_norm), _p(zz), nz, niter, _p(poly),
+                                    len(poly) - 1, _p(spec), blksz, int(tda), _p(a), _p(b))
+    return spec, a, b
+
+
+def pvectors(psi, mmat, sqrtR, m, n0, blksz, nv, nc, tda=False):
+    psi, mmat, sqrtR = _f64(psi), _f64(mmat), _f64(sqrtR)
+    nmu = psi.shape[0]
+    p = np.zeros((nv * nc, blksz), order="F")
+    pn2 = np.zeros(blksz)
+    # cm(mu, icv), icv = iv + nv*ic: the (nmu, nc, nv) product reshaped in C order
+    cm = np.asfortranarray((psi[:, nv:nv + nc][:, :, None] * psi[:, :nv][:, None, :]).reshape(nmu, nc * nv))
+    lib().orc_pvectors(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0], _p(sqrtR),
+                       int(m), int(n0), int(blksz), int(tda), _p(cm), _p(p), _p(pn2))
+    return p, pn2
+
+
+def sigma_lanczos(psi, mmat, e1, nv, nc, qp, nen, deltae, ecuts, niter, poly, jnblock, tda=False):
+    """Sigma_c(omega) (nen, nqp) complex in Ry for 0-based QP band indices `qp`."""
+    psi, mmat, e1, poly = _f64(psi), _f64(mmat), _f64(e1), _f64(poly)
+    qp = np.ascontiguousarray(qp, np.int32)
+    out = np.zeros((nen, len(qp)), np.complex128, order="F")
+    nmu = psi.shape[0]
+    lib().orc_sigma_lanczos(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0], _p(e1),
+                            len(qp), qp.ctypes.data_as(_ip), nen, float(deltae), float(ecuts),
+                            int(niter), _p(poly), len(poly) - 1, int(jnblock), int(tda), _p(out))
+    return out
+
+
+def spline_evaluate(xtable, ftable, xx):
+    xtable, ftable = _f64(xtable), _c128(ftable)
+    fx = np.zeros(1, np.complex128)
+    df = np.zeros(1)
+    lib().orc_spline_evaluate(len(xtable), _p(xtable), _p(ftable), float(xx), _p(fx), _p(df))
+    return complex(fx[0]), float(df[0])
+
+
+def newtonr(xtable, ftable, x0, xacc=0.01):
+    xtable, ftable = _f64(xtable), _c128(ftable)
+    x = np.array([x0], np.float64)
+    fx = np.zeros(1, np.complex128)
+    ierr = lib().orc_newtonr(len(xtable), _p(xtable), _p(ftable), _p(x), float(xacc), _p(fx))
+    return float(x[0]), complex(fx[0]), ierr
+
+
+def qp_solve(sigc_ev, deltae_ev, edft, en0, sigx, vxc):
+    """printsigma.F90:97-128 for one state; all energies in eV."""
+    sigc = _c128(sigc_ev)
+    out = np.zeros(3)
+    s0 = np.zeros(1, np.complex128)
+    e = np.zeros(1); z = np.zeros(1); im = np.zeros(1)
+    ierr = lib().orc_qp_solve(len(sigc), float(deltae_ev), float(edft), float(en0), float(sigx),
+                              float(vxc), _p(sigc), _p(e), _p(z), _p(s0), _p(im))
+    del out
+    return {"eqp": float(e[0]), "zfac": float(z[0]), "sigma0": complex(s0[0]),
+            "imsig": float(im[0]), "ierr": int(ierr)}
+
+
+def polyfit_sqrt(xmin, xmax, npoly, nx=20, cheb=False):
+    """Returns (beta low->high, fit_error); raises on the reference's die() ranges."""
+    beta = np.zeros(npoly + 1)
+    fe = np.zeros(1)
+    rc = lib().orc_polyfit_sqrt(float(xmin), float(xmax), int(npoly), int(nx), int(cheb), _p(beta),
+                                _p(fe))
+    if rc != 0:
+        raise ValueError("polyfit_sqrt: xmin too small")
+    return beta, float(fe[0])
+
+
+# ---------------------------------------------------------------------------
+# FFT Poisson solve and ISDF Coulomb kernel (numpy restatement)
+# ---------------------------------------------------------------------------
+
+QZERO = 1.0e-6  # shared/fft_module.F90z:63
+
+
+def coulomb_0d(nfft, h, trunc):
+    """v(G) on the rfftn half box, shared/fft_module.F90z:628-691 (Ry, e^2 = 2)."""
+    n1, n2, n3 = nfft
+    b = [2.0 * np.pi / (n * h) for n in nfft]  # parsec_wfn.F90 bvec(ii,ii) = 2 pi/(nfft h)
+    j1 = np.fft.fftfreq(n1, 1.0 / n1)[: n1 // 2 + 1]
+    j1 = np.abs(j1)
+    j2 = np.fft.fftfreq(n2, 1.0 / n2)
+    j3 = np.fft.fftfreq(n3, 1.0 / n3)
+    g2 = ((b[0] * j1)[None, None, :] ** 2 + (b[1] * j2)[None, :, None] ** 2 +
+          (b[2] * j3)[:, None, None] ** 2)
+    # axes ordered (z, y, x) so that rfftn's last axis is x
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if trunc > 0:
+            v = np.where(g2 < QZERO * QZERO, trunc ** 2 / 2.0,
+                         (1.0 - np.cos(trunc * np.sqrt(g2))) / g2)
+        else:
+            v = np.where(g2 < QZERO * QZERO, 0.0, 1.0 / g2)
+    return 8.0 * np.pi * v
+
+
+def poisson(rho, gidx, nfft, h, trunc, vcoul=None):
+    """V(r) = IFFT(FFT(rho) v(G)) / N on the grid points (fft_module.F90z:122-177).
+
+    rho: (ngrid,) or (ngrid, nvec); gidx: (ngrid, 3) integer coordinates in
+    [-nfft/2, nfft/2).
+    """
+    if vcoul is None:
+        vcoul = coulomb_0d(nfft, h, trunc)
+    rho = np.asarray(rho, np.float64)
+    single = rho.ndim == 1
+    if single:
+        rho = rho[:, None]
+    n1, n2, n3 = nfft
+    ix = np.mod(gidx[:, 0], n1)
+    iy = np.mod(gidx[:, 1], n2)
+    iz = np.mod(gidx[:, 2], n3)
+    out = np.empty_like(rho)
+    for k in range(rho.shape[1]):
+        box = np.zeros((n3, n2, n1))
+        box[iz, iy, ix] = rho[:, k]
+        f = np.fft.rfftn(box)
+        f *= vcoul
+        box = np.fft.irfftn(f, s=(n3, n2, n1))
+        out[:, k] = box[iz, iy, ix]
+    return out[:, 0] if single else out
+
+
+def isdf_kernel(zeta, gidx, nfft, h, trunc, ntrans=1):
+    """M = (ntrans/h^3) zeta^T (V zeta), isdf_parallel_sym_UltraLowMem.F90:312, 749, 865-867."""
+    vz = poisson(zeta, gidx, nfft, h, trunc)
+    return (ntrans / h ** 3) * (zeta.T @ vz)

@@ -1,0 +1,165 @@
+"""GPU parity of the FP64 HIP path against the CPU oracle (through the C ABI).
+
+Tolerances: the FP64 operator, Horner and Lanczos coefficients agree with the
+oracle to round-off (relative 1e-10 or better, summation order differs);
+Sigma_c(omega) within 1e-6 relative and QP energies within 1 meV, the bars
+NanoGW's FP64 path must meet.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+RYD = 13.60569193
+
+
+def _poly(mol, npoly=12):
+    from nanogw_amd.lanczos import lanczos_polynomial
+
+    return lanczos_polynomial(mol.e1, mol.nv, npoly)
+
+
+def test_gemm_f64_lane_layout(gpu, ngw_ctx):
+    """The MFMA f64 16x16x4 GEMM reproduces A @ B on ragged shapes."""
+    import torch
+
+    rng = np.random.default_rng(0)
+    for (m, n, k) in [(16, 16, 4), (70, 45, 33), (64, 64, 16), (129, 3, 200), (1, 1, 1)]:
+        a = rng.normal(size=(m, k))
+        b = rng.normal(size=(k, n))
+        ta = torch.tensor(a.T.copy(), dtype=torch.float64, device=gpu)  # column-major a
+        tb = torch.tensor(b.T.copy(), dtype=torch.float64, device=gpu)
+        tc = torch.zeros((n, m), dtype=torch.float64, device=gpu)
+        torch.cuda.synchronize()
+        ngw_ctx.gemm_f64_dev(m, n, k, ta.data_ptr(), m, tb.data_ptr(), k, tc.data_ptr(), m)
+        c = tc.cpu().numpy().T
+        assert rel_err(c, a @ b) < 1e-13, (m, n, k)
+
+
+@pytest.mark.parametrize("tda", [False, True])
+@pytest.mark.parametrize("blksz", [1, 3, 17])
+def test_matvec_block_vs_oracle(ngw_ctx, mini, tda, blksz):
+    from oracle import oracle as O
+
+    ngw_ctx.set_isdf(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv)
+    ncv = mini.nv * mini.nc
+    x = np.random.default_rng(blksz).normal(size=(ncv, blksz))
+    y = ngw_ctx.matvec_isdf_BLOCK(x, tamm_d=tda)
+    s = O.sqrt_r(mini.e1, mini.nv, mini.nc)
+    ref = O.matvec_block(mini.psi_intp, mini.mmtrx, s, x, mini.nv, mini.nc, tda)
+    print("matvec", tda, blksz, rel_err(y, ref))
+    assert rel_err(y, ref) < 1e-11
+
+
+def test_polynomial_matvec_vs_oracle(ngw_ctx, mini):
+    from oracle import oracle as O
+
+    ngw_ctx.set_isdf(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv)
+    poly = _poly(mini)
+    x = np.random.default_rng(5).normal(size=(mini.nv * mini.nc, 4))
+    y = ngw_ctx.polynomial_matvec_isdf_BLOCK(x, poly)
+    s = O.sqrt_r(mini.e1, mini.nv, mini.nc)
+    ref = O.polynomial_matvec_block(mini.psi_intp, mini.mmtrx, s, x, mini.nv, mini.nc, poly)
+    assert rel_err(y, ref) < 1e-9
+
+
+def test_lanczos_spectra_vs_oracle(ngw_ctx, mini):
+    from oracle import oracle as O
+
+    ngw_ctx.set_isdf(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv)
+    poly = _poly(mini)
+    ncv = mini.nv * mini.nc
+    rng = np.random.default_rng(9)
+    v0 = rng.normal(size=(ncv, 5))
+    vn = np.linalg.norm(v0, axis=0)
+    zz = (rng.uniform(-2, 4, size=(31, 5)) + 0.05j)
+    zz[-1, :] = 0.0
+    niter = 25
+    spec, a, b = ngw_ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, niter, poly)
+    s = O.sqrt_r(mini.e1, mini.nv, mini.nc)
+    rs, ra, rb = O.lanczos_spectra_block(mini.psi_intp, mini.mmtrx, s, v0, vn, zz, niter, poly,
+                                         mini.nv, mini.nc)
+    assert rel_err(a[:6], ra[:6]) < 1e-10
+    assert rel_err(spec, rs) < 1e-6
+
+
+def test_sigma_lanczos_mini_vs_oracle(ngw_ctx, mini):
+    from oracle import oracle as O
+
+    ngw_ctx.set_isdf(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv)
+    poly = _poly(mini)
+    qp = [mini.nv - 1, mini.nv]
+    nen, deltae, ecuts, niter = 41, 4.0, 0.02, 30
+    sig = ngw_ctx.calculate_sigma_lanczos(qp, nen, deltae, ecuts, niter, poly)
+    ref = O.sigma_lanczos(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv, mini.nc, qp, nen, deltae, ecuts,
+                          niter, poly, 7)
+    print("sigma mini", rel_err(sig, ref))
+    assert rel_err(sig, ref) < 1e-6
+
+
+def test_sigma_lanczos_tda_vs_oracle(ngw_ctx, mini_tda):
+    from oracle import oracle as O
+
+    m = mini_tda
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    qp = [m.nv - 1, m.nv]
+    poly = np.array([1.0, 0.0])  # TDA: p(x) = x (calculate_sigma_lanczos_lowcomsym.F90z:242-245)
+    sig = ngw_ctx.calculate_sigma_lanczos(qp, 21, 3.0, 0.02, 20, poly, tamm_d=True)
+    ref = O.sigma_lanczos(m.psi_intp, m.mmtrx, m.e1, m.nv, m.nc, qp, 21, 3.0, 0.02, 20, poly, 5,
+                          tda=True)
+    assert rel_err(sig, ref) < 1e-6
+
+
+def test_sigma_c1_fp64_parity(ngw_ctx, c1_inputs, c1_sigma):
+    """Config 1: Sigma_c within 1e-6 relative and QP energies within 1 meV of the oracle."""
+    d = c1_inputs
+    ngw_ctx.set_isdf(d["psi_intp"], d["mmtrx"], d["e1"], int(d["nv"]))
+    qp = [int(x) for x in d["qp"]]
+    sig = ngw_ctx.calculate_sigma_lanczos(qp, int(d["nen"]), float(d["deltae"]), float(d["ecuts"]),
+                                          int(c1_sigma["niter"]), d["poly"])
+    ref = c1_sigma["sigma"]
+    err = rel_err(sig, ref)
+    print("c1 sigma rel err", err)
+    assert err < 1e-6
+    e1 = d["e1"]
+    res = ngw_ctx.qp_newton(sig * RYD, float(d["deltae"]) * RYD, e1[qp] * RYD, e1[qp] * RYD,
+                            d["sigx"] * RYD, np.zeros(len(qp)))
+    ref_qp = c1_sigma["qp_results"]
+    dq = np.abs(res["eqp"] - ref_qp[:, 0]).max()
+    print("c1 QP |dE| (eV)", dq, res["eqp"])
+    assert dq < 1e-3
+    assert np.all(res["ierr"] == ref_qp[:, 4])
+
+
+def test_qp_newton_vs_oracle(ngw_ctx):
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(3)
+    nen, nqp = 200, 5
+    x = np.linspace(-1, 1, nen)
+    sig = np.empty((nen, nqp), np.complex128)
+    for i in range(nqp):
+        c = rng.normal(size=4)
+        sig[:, i] = (c[0] + c[1] * x + c[2] * np.sin(3 * x) + 0.3 * c[3] * x ** 3) + 1j * (0.1 * np.cos(x))
+    edft = rng.uniform(-10, 0, nqp)
+    sigx = rng.uniform(-5, 0, nqp)
+    vxc = rng.uniform(-3, 0, nqp)
+    res = ngw_ctx.qp_newton(sig * 13.6, 27.2, edft, edft, sigx, vxc)
+    for i in range(nqp):
+        r = O.qp_solve(sig[:, i] * 13.6, 27.2, edft[i], edft[i], sigx[i], vxc[i])
+        assert abs(res["eqp"][i] - r["eqp"]) < 1e-9
+        assert abs(res["zfac"][i] - r["zfac"]) < 1e-9
+        assert abs(res["imsig"][i] - r["imsig"]) < 1e-9
+        assert res["ierr"][i] == r["ierr"]
+
+
+def test_errors_raise(ngw_ctx, mini):
+    from nanogw_amd._lib import NgwError
+
+    ngw_ctx.set_isdf(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv)
+    with pytest.raises(NgwError):
+        ngw_ctx.lanczos_spectra_isdf_BLOCK(np.ones((7, 1)), [1.0], np.zeros((3, 1)), 5, [1.0, 0.0])
+    with pytest.raises(NgwError):
+        ngw_ctx.calculate_sigma_lanczos([999], 10, 1.0, 0.01, 5, [1.0, 0.0])
