@@ -183,6 +183,20 @@ int ngw_matvec_isdf_block_dev(ngw_context* ctx, const double* d_vec, double* d_h
 int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, long lda,
                      const double* d_b, long ldb, double* d_c, long ldc);
 
+/*
+ * Per-launch kernel timing with HIP events on the context's stream (no
+ * NanoGW counterpart; NanoGW's timacc slots 68-78 time the same stages on the
+ * host, shared/apply_matvec_BLOCK.F90:52-206).  Kernel classes:
+ * 0 K1 (C x GEMM), 1 K2 (M t GEMM), 2 K3 (C^T w GEMM + epilogue),
+ * 3 Lanczos alpha/beta, 4 continued fraction, 5 P-vector build,
+ * 6 Sigma_c assembly.  ngw_profile_read synchronizes and returns the summed
+ * milliseconds and launch count of one class since the last reset.
+ */
+#define NGW_KC_COUNT 7
+int ngw_profile_enable(ngw_context* ctx, int on);
+int ngw_profile_reset(ngw_context* ctx);
+int ngw_profile_read(ngw_context* ctx, int kernel_class, double* total_ms, long* launches);
+
 #ifdef __cplusplus
 }
 #endif

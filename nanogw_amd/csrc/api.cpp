@@ -47,6 +47,43 @@ void DevBuf::release()
     bytes = 0;
 }
 
+ProfScope::ProfScope(ngw_context* ctx, int k) : c(ctx), kind(k)
+{
+    if (!c || !c->prof_on) return;
+    for (hipEvent_t* e : {&a, &b}) {
+        if (!c->prof_pool.empty()) {
+            *e = c->prof_pool.back();
+            c->prof_pool.pop_back();
+        } else if (hipEventCreate(e) != hipSuccess) {
+            *e = nullptr;
+        }
+    }
+    if (a) (void)hipEventRecord(a, c->stream);
+}
+
+ProfScope::~ProfScope()
+{
+    if (!c || !c->prof_on || !a || !b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->prof_recs.push_back({kind, a, b});
+}
+
+static void prof_drain(ngw_context* c)
+{
+    if (c->prof_recs.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& r : c->prof_recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            c->prof_ms[r.kind] += ms;
+            c->prof_cnt[r.kind] += 1;
+        }
+        c->prof_pool.push_back(r.a);
+        c->prof_pool.push_back(r.b);
+    }
+    c->prof_recs.clear();
+}
+
 int operator_apply(ngw_context* ctx, const OperatorArgs& a)
 {
     if (ctx->precision == PREC_F64) return operator_f64(ctx, a);
@@ -188,6 +225,11 @@ int ngw_destroy(ngw_context* ctx)
                       &ctx->tpart, &ctx->ubuf, &ctx->w0, &ctx->w1, &ctx->w2, &ctx->w3, &ctx->w4,
                       &ctx->ab, &ctx->zz, &ctx->spec, &ctx->misc, &ctx->misc2, &ctx->staging};
     for (DevBuf* b : bufs) b->release();
+    for (auto& r : ctx->prof_recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return NGW_OK;
@@ -437,11 +479,15 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
         NGW_TRY(scale_columns(s, q, q, ncv, ncv, nc, pn + c0));
         NGW_TRY(lanczos_run(ctx, q, w, ctx->w2.as<double>(), ctx->w3.as<double>(), ncv, nc, niter,
                             polynomial, npoly, tamm_d ? 1 : 0, da, db));
+        ProfScope ps(ctx, KC_CFRAC);
         NGW_TRY(continued_fraction(s, da, db, niter, nc, zzd + 2 * (size_t)nz * c0, nz,
                                    spec + 2 * (size_t)nz * c0));
     }
-    NGW_TRY(sigma_assemble(s, spec, zzd, pn2, nen, nqp, nb, ctx->nval, tamm_d ? 1 : 0,
-                           ctx->misc2.as<double>()));
+    {
+        ProfScope ps(ctx, KC_SIGMA);
+        NGW_TRY(sigma_assemble(s, spec, zzd, pn2, nen, nqp, nb, ctx->nval, tamm_d ? 1 : 0,
+                               ctx->misc2.as<double>()));
+    }
     NGW_HIP(hipMemcpyAsync(sigma_c, ctx->misc2.ptr, sizeof(double) * 2 * (size_t)nen * nqp,
                            hipMemcpyDeviceToHost, s));
     NGW_HIP(hipStreamSynchronize(s));
@@ -538,6 +584,34 @@ int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, l
     NGW_HIP(hipSetDevice(ctx->device));
     NGW_TRY(gemm_f64(ctx->stream, m, n, k, d_a, lda, d_b, ldb, d_c, ldc));
     NGW_HIP(hipStreamSynchronize(ctx->stream));
+    return NGW_OK;
+}
+
+int ngw_profile_enable(ngw_context* ctx, int on)
+{
+    if (!ctx) return fail(NGW_ERR_ARG, "null context");
+    ctx->prof_on = on != 0;
+    return NGW_OK;
+}
+
+int ngw_profile_reset(ngw_context* ctx)
+{
+    if (!ctx) return fail(NGW_ERR_ARG, "null context");
+    prof_drain(ctx);
+    for (int k = 0; k < KC_COUNT; ++k) {
+        ctx->prof_ms[k] = 0.0;
+        ctx->prof_cnt[k] = 0;
+    }
+    return NGW_OK;
+}
+
+int ngw_profile_read(ngw_context* ctx, int kernel_class, double* total_ms, long* launches)
+{
+    if (!ctx || !total_ms || !launches) return fail(NGW_ERR_ARG, "null argument");
+    if (kernel_class < 0 || kernel_class >= KC_COUNT) return fail(NGW_ERR_ARG, "bad kernel class");
+    prof_drain(ctx);
+    *total_ms = ctx->prof_ms[kernel_class];
+    *launches = ctx->prof_cnt[kernel_class];
     return NGW_OK;
 }
 

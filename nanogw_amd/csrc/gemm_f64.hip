@@ -52,7 +52,8 @@ __device__ __forceinline__ void load_tile(const L& ld, int mn0, int k0, int K, d
 }
 
 // acc[i][j] holds the 16x16 block (rows wm+16i.., cols wn+16j..) of this wave;
-// lane l, component r <-> row 4*(l/16)+r, col l%16 (v_mfma_f64_16x16x4f64).
+// lane l, component r <-> row l/16 + 4r, col l%16 (v_mfma_f64_16x16x4f64 output
+// layout, measured on gfx950; A/B operands: lane l holds row/col l%16, k = l/16).
 template <class AL, class BL>
 __device__ __forceinline__ void mainloop(const AL& al, const BL& bl, int m0, int n0, int K,
                                          f64x4 (&acc)[2][2], SmemAB* sm)
@@ -95,7 +96,7 @@ __device__ __forceinline__ void for_acc(const f64x4 (&acc)[2][2], F&& f)
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                f(wm + 16 * i + 4 * (lane >> 4) + r, wn + 16 * j + (lane & 15), acc[i][j][r]);
+                f(wm + 16 * i + (lane >> 4) + 4 * r, wn + 16 * j + (lane & 15), acc[i][j][r]);
 }
 
 // ---------------------------------------------------------------- plain GEMM
@@ -131,6 +132,38 @@ int gemm_f64(hipStream_t s, int m, int n, int k, const double* a, long lda, cons
     int mt = (m + BM - 1) / BM, nt = (n + BN - 1) / BN;
     hipLaunchKernelGGL(gemm_f64_kernel, dim3(mt * nt), dim3(NT), 0, s, m, n, k, a, lda, b, ldb, c,
                        ldc, mt);
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
+}
+
+// C(m, n) = alpha A(k, m)^T B(k, n): A read along k (zeta^T V zeta of the V build)
+struct ColMajorAT {
+    static constexpr bool kContig = true;
+    const double* a; long ld; int MN;
+    __device__ double operator()(int m, int k) const { return a[k + (long)m * ld]; }
+};
+
+__global__ __launch_bounds__(NT) void gemm_f64_tn_kernel(int m, int n, int k, double alpha, const double* a,
+                                                         long lda, const double* b, long ldb, double* c,
+                                                         long ldc, int mtiles)
+{
+    __shared__ SmemAB sm;
+    const int m0 = (blockIdx.x % mtiles) * BM, n0 = (blockIdx.x / mtiles) * BN;
+    f64x4 acc[2][2];
+    mainloop(ColMajorAT{a, lda, m}, ColMajorB{b, ldb, n}, m0, n0, k, acc, &sm);
+    for_acc(acc, [&](int row, int col, double val) {
+        int gm = m0 + row, gn = n0 + col;
+        if (gm < m && gn < n) c[gm + (long)gn * ldc] = alpha * val;
+    });
+}
+
+int gemm_f64_tn(hipStream_t s, int m, int n, int k, double alpha, const double* a, long lda,
+                const double* b, long ldb, double* c, long ldc)
+{
+    if (m <= 0 || n <= 0) return NGW_OK;
+    int mt = (m + BM - 1) / BM, nt = (n + BN - 1) / BN;
+    hipLaunchKernelGGL(gemm_f64_tn_kernel, dim3(mt * nt), dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb,
+                       c, ldc, mt);
     NGW_HIP(hipGetLastError());
     return NGW_OK;
 }
@@ -280,6 +313,7 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a)
     hipStream_t s = ctx->stream;
     const double* sx = a.tda ? nullptr : ctx->sqrtr.as<double>();
     {
+        ProfScope ps(ctx, KC_K1);
         int mt = (nmu + BM - 1) / BM, nt = (nv * B + BN - 1) / BN;
         hipLaunchKernelGGL(k1_f64_kernel, dim3(mt * nt), dim3(NT), 0, s,
                            K1A{psic, ldp, nmu}, K1B{a.x, sx, a.ldx, nv, nv * B}, psiv, ldp, nmu, nc,
@@ -287,6 +321,7 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a)
         NGW_HIP(hipGetLastError());
     }
     {
+        ProfScope ps(ctx, KC_K2);
         int mt = (nmu + BM - 1) / BM, nt = (B + BN - 1) / BN;
         hipLaunchKernelGGL(k2_f64_kernel<K2BPart>, dim3(mt * nt), dim3(NT), 0, s,
                            K2A{ctx->mmat.as<double>(), ctx->ld_m, nmu},
@@ -294,6 +329,7 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a)
         NGW_HIP(hipGetLastError());
     }
     {
+        ProfScope ps(ctx, KC_K3);
         OperatorEpilogue ep = a.epilogue(ctx);
         int mt = (nc + BM - 1) / BM, nt = (nv * B + BN - 1) / BN;
         hipLaunchKernelGGL(k3_f64_kernel, dim3(mt * nt), dim3(NT), 0, s, K3A{psic, ldp, nc},
@@ -314,6 +350,7 @@ int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int n
     NGW_TRY(ctx->ubuf.ensure(sizeof(double) * (size_t)B * nmu));
     double* u = ctx->ubuf.as<double>();
     hipStream_t s = ctx->stream;
+    ProfScope ps(ctx, KC_PVEC);
     {
         int mt = (nmu + BM - 1) / BM, nt = (B + BN - 1) / BN;
         hipLaunchKernelGGL(k2_f64_kernel<K2BPair>, dim3(mt * nt), dim3(NT), 0, s,

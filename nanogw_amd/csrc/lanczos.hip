@@ -128,6 +128,7 @@ int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double
     const long n = ctx->ncv;
     for (int k = 1; k <= niter; ++k) {
         NGW_TRY(horner(ctx, d_q, d_w, d_h0, d_h1, ld, nvec, poly, npoly, tda, k > 1));
+        ProfScope ps(ctx, KC_LANCZOS);
         hipLaunchKernelGGL(lanczos_ab_kernel, dim3(nvec), dim3(RT), 0, s, d_q, d_w, ld, n, d_a, d_b,
                            niter, k);
         NGW_HIP(hipGetLastError());

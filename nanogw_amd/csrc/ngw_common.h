@@ -39,6 +39,9 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Kernel classes timed by the optional per-launch HIP-event profiler.
+enum KernelClass { KC_K1 = 0, KC_K2, KC_K3, KC_LANCZOS, KC_CFRAC, KC_PVEC, KC_SIGMA, KC_COUNT };
+
 // Precision of the three GEMM stages of the operator.
 enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2 };
 
@@ -68,4 +71,20 @@ struct ngw_context {
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;
     // host staging for drop-in calls
     std::vector<double> hbuf;
+    // per-launch event profiler (off unless ngw_profile_enable)
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_pool;
+    struct ProfRec { int kind; hipEvent_t a, b; };
+    std::vector<ProfRec> prof_recs;
+    double prof_ms[ngw::KC_COUNT] = {0};
+    long prof_cnt[ngw::KC_COUNT] = {0};
 };
+
+namespace ngw {
+// Records HIP events around one kernel launch on ctx->stream when profiling.
+struct ProfScope {
+    ngw_context* c; int kind; hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(ngw_context* ctx, int k);
+    ~ProfScope();
+};
+}  // namespace ngw

@@ -63,6 +63,8 @@ struct OperatorArgs {
 
 int gemm_f64(hipStream_t s, int m, int n, int k, const double* a, long lda, const double* b,
              long ldb, double* c, long ldc);
+int gemm_f64_tn(hipStream_t s, int m, int n, int k, double alpha, const double* a, long lda,
+                const double* b, long ldb, double* c, long ldc);
 int k1_slots(int nv);
 int operator_f64(ngw_context* ctx, const OperatorArgs& a);
 int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int nvec, int tda,

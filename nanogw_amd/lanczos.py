@@ -190,5 +190,24 @@ class IsdfContext:
         _lib.check(self._lib.ngw_gemm_f64_dev(self._h, m, n, k, d_a, lda, d_b, ldb, d_c, ldc),
                    "ngw_gemm_f64_dev")
 
+    # ------------------------------------------------------------ profiling
+    KERNEL_CLASSES = ("K1", "K2", "K3", "lanczos_ab", "cfrac", "pvectors", "sigma")
+
+    def profile_enable(self, on=True):
+        _lib.check(self._lib.ngw_profile_enable(self._h, int(bool(on))), "ngw_profile_enable")
+
+    def profile_reset(self):
+        _lib.check(self._lib.ngw_profile_reset(self._h), "ngw_profile_reset")
+
+    def profile_read(self):
+        """{class: (total_ms, launches)} since the last reset (synchronizes)."""
+        out = {}
+        for k, name in enumerate(self.KERNEL_CLASSES):
+            ms = C.c_double()
+            n = C.c_long()
+            _lib.check(self._lib.ngw_profile_read(self._h, k, C.byref(ms), C.byref(n)), "ngw_profile_read")
+            out[name] = (ms.value, n.value)
+        return out
+
     def synchronize(self):
         _lib.check(self._lib.ngw_synchronize(self._h), "ngw_synchronize")

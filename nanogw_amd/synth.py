@@ -20,7 +20,8 @@ energies, the ISDF interpolation vectors zeta and the Coulomb kernel
   (shared/isdf_parallel_sym_UltraLowMem.F90:448-642).
 
 The Coulomb kernel M = (1/h^3) zeta^T V zeta is built by the device path
-(``nanogw_amd.isdf.coulomb_kernel``) in production; ``host_coulomb_kernel``
+(``IsdfContext.isdf_coulomb_kernel``, the device V build) when a context is
+passed; ``host_coulomb_kernel``
 below is a small numpy version used only to prepare inputs of CPU-side tests
 when no GPU is present.
 
@@ -167,18 +168,20 @@ def cvt_points(gidx, h, rho, n_intp, rng, tol=1e-3, max_iter=300):
                         for k in range(3)], axis=1)
         ok = w >= 1e-10
         cand = np.where(ok[:, None], cen / np.where(ok, w, 1.0)[:, None], old)
-        taken = set()
+        _, near = tree_grid.query(cand)
+        taken = np.zeros(pts.shape[0], bool)
         for ip in range(n_intp):
-            k = 1
-            while True:
-                _, idx = tree_grid.query(cand[ip], k=k)
-                idx = np.atleast_1d(idx)
-                free = [i for i in idx if i not in taken]
-                if free:
-                    break
-                k *= 2
-            g = int(free[0])
-            taken.add(g)
+            g = int(near[ip])
+            if taken[g]:
+                k = 2
+                while True:
+                    _, idx = tree_grid.query(cand[ip], k=k)
+                    free = [i for i in np.atleast_1d(idx) if not taken[i]]
+                    if free:
+                        g = int(free[0])
+                        break
+                    k *= 2
+            taken[g] = True
             intp[ip] = g
         new = pts[intp]
         diff = np.sqrt(((new - old) ** 2).sum(axis=1)).sum()
@@ -188,8 +191,30 @@ def cvt_points(gidx, h, rho, n_intp, rng, tol=1e-3, max_iter=300):
     return intp
 
 
-def isdf_zeta(psi, intp, nv, ridge=0.0):
-    """zeta (ngrid, n_intp) = (A^{-1} B)^T, isdf_parallel_sym_UltraLowMem.F90:448-642."""
+def isdf_zeta(psi, intp, nv, ridge=0.0, device=None):
+    """zeta (ngrid, n_intp) = (A^{-1} B)^T, isdf_parallel_sym_UltraLowMem.F90:448-642.
+
+    device: a torch device to run the FP64 products and the solve on (input
+    preparation of large synthetic systems); numpy otherwise.
+    """
+    if device is not None:
+        import torch
+
+        t = torch.as_tensor(np.asarray(psi), dtype=torch.float64, device=device)
+        ii = torch.as_tensor(np.asarray(intp), dtype=torch.long, device=device)
+        pv = t[:, :nv]
+        pi_v = pv[ii, :]
+        pi_n = t[ii, :]
+        A = (pi_v @ pi_v.T) * (pi_n @ pi_n.T)
+        if ridge > 0.0:
+            A = A + ridge * torch.trace(A) / A.shape[0] * torch.eye(A.shape[0], dtype=A.dtype, device=device)
+        B = (pv @ pi_v.T) * (t @ pi_n.T)
+        del t, pv
+        z = torch.linalg.solve(A, B.T).T
+        out = np.asfortranarray(z.cpu().numpy())
+        del A, B, z
+        torch.cuda.empty_cache()
+        return out
     pv = psi[:, :nv]
     pi_v = pv[intp, :]
     pi_n = psi[intp, :]
@@ -225,13 +250,17 @@ def host_coulomb_kernel(zeta, gidx, nfft, h, trunc, ntrans=1):
         box = np.zeros((n3, n2, n1))
         box[iz, iy, ix] = zeta[:, k]
         f = np.fft.rfftn(box) * v
-        vz[:, k] = np.fft.irfftn(f, s=(n3, n2, n1))[iz, iy, ix]
+        vz[:, k] = np.fft.irfftn(f, s=(n3, n2, n1), axes=(0, 1, 2))[iz, iy, ix]
     return (ntrans / h ** 3) * (zeta.T @ vz)
 
 
 def make_molecule(radius, h, nv, nc, n_intp, seed, with_kernel=True, keep_psi=False,
-                  cvt_iter=300):
-    """Build one synthetic molecule; see the module docstring for conventions."""
+                  cvt_iter=300, ridge=0.0, kernel_ctx=None, device=None, keep_zeta=True):
+    """Build one synthetic molecule; see the module docstring for conventions.
+
+    kernel_ctx: an ``IsdfContext`` whose device V build computes M (otherwise
+    the numpy ``host_coulomb_kernel``); device: torch device for the zeta solve.
+    """
     rng = np.random.default_rng(seed)
     gidx = sphere_grid(radius, h)
     rmax, nfft, trunc = fft_box(gidx, h)
@@ -244,12 +273,31 @@ def make_molecule(radius, h, nv, nc, n_intp, seed, with_kernel=True, keep_psi=Fa
     mol = Molecule(radius, h, nv, nc, n_intp, seed, gidx, psi if keep_psi else None, e1, intp,
                    psi_intp, nfft, rmax, trunc)
     if with_kernel:
-        mol.zeta = isdf_zeta(psi, intp, nv)
-        mol.mmtrx = np.asfortranarray(host_coulomb_kernel(mol.zeta, gidx, nfft, h, trunc))
-        mol.mmtrx = 0.5 * (mol.mmtrx + mol.mmtrx.T)
+        zeta = isdf_zeta(psi, intp, nv, ridge=ridge, device=device)
+        if kernel_ctx is not None:
+            m = kernel_ctx.isdf_coulomb_kernel(zeta, gidx, nfft, h, trunc)
+        else:
+            m = host_coulomb_kernel(zeta, gidx, nfft, h, trunc)
+        mol.mmtrx = np.asfortranarray(0.5 * (m + m.T))
+        if keep_zeta:
+            mol.zeta = zeta
     if keep_psi:
         mol.psi = psi
     return mol
+
+
+CONFIGS = {
+    # name: (R, h, nv, nc, n_intp, niter, number of QP states)
+    "c1": (7.0, 0.45, 8, 56, 480, 200, 4),
+    "c2": (10.0, 0.35, 40, 400, 2000, 300, 8),
+    "c3": (13.0, 0.33, 100, 1500, 5000, 300, 8),
+}
+
+
+def qp_states(nv, nqp):
+    """nqp QP states centred on the gap: HOMO-(nqp/2-1) .. LUMO+(nqp/2-1) (0-based)."""
+    half = nqp // 2
+    return list(range(nv - half, nv + half))
 
 
 def config_seed(config: int, index: int = 0) -> int:

@@ -318,11 +318,16 @@ void orc_lanczos_spectra_block(int nmu, int nv, int nc, const double *psi, long 
                                const double *v0, const double *v0_norm,
                                const double complex *zz, int nz, int niter,
                                const double *poly, int npoly, double complex *spectra,
-                               int blksz, int tda, double *a_out, double *b_out)
+                               int blksz, int tda, double *a_out, double *b_out, int rebuild)
 {
+    /* rebuild != 0: rebuild Cmtrx inside every operator call, as
+       matvec_isdf_BLOCK does (apply_matvec_BLOCK.F90:99-107, :170-178) */
     long ncv = (long)nv * nc;
-    double *cm = (double *)malloc(sizeof(double) * ncv * nmu);
-    build_cmtrx(nmu, nv, nc, psi, ldpsi, cm);
+    double *cm = NULL;
+    if (!rebuild) {
+        cm = (double *)malloc(sizeof(double) * ncv * nmu);
+        build_cmtrx(nmu, nv, nc, psi, ldpsi, cm);
+    }
     struct isdf_ctx c = {nmu, nv, nc, psi, ldpsi, mmat, ldm, sqrtR, poly, npoly, tda, cm};
     lanczos_block_generic(isdf_apply, &c, v0, v0_norm, ncv, zz, nz, niter, spectra, blksz,
                           a_out, b_out);

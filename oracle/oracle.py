@@ -61,7 +61,8 @@ def lib():
                                         _dp, _dp, _dp]
         L.orc_lanczos_spectra_block.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp,
                                                 C.c_long, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
-                                                _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp]
+                                                _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
+                                                C.c_int]
         L.orc_pvectors.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long, _dp,
                                    C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_sigma_lanczos.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
@@ -141,7 +142,8 @@ def lanczos_dense(h, v0, zz, niter):
     return spec, a, b
 
 
-def lanczos_spectra_block(psi, mmat, sqrtR, v0, v0_norm, zz, niter, poly, nv, nc, tda=False):
+def lanczos_spectra_block(psi, mmat, sqrtR, v0, v0_norm, zz, niter, poly, nv, nc, tda=False,
+                          rebuild=False):
     """zz: (nz, blksz) complex.  Returns spectra (nz, blksz), a, b (niter, blksz)."""
     psi, mmat, sqrtR, poly = _f64(psi), _f64(mmat), _f64(sqrtR), _f64(poly)
     v0 = _f64(v0.reshape(nv * nc, -1, order="F"))
@@ -155,7 +157,8 @@ def lanczos_spectra_block(psi, mmat, sqrtR, v0, v0_norm, zz, niter, poly, nv, nc
     nmu = psi.shape[0]
     lib().orc_lanczos_spectra_block(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0],
                                     _p(sqrtR), _p(v0), _p(v0_norm), _p(zz), nz, niter, _p(poly),
-                                    len(poly) - 1, _p(spec), blksz, int(tda), _p(a), _p(b))
+                                    len(poly) - 1, _p(spec), blksz, int(tda), _p(a), _p(b),
+                                    int(rebuild))
     return spec, a, b
 
 
@@ -272,7 +275,7 @@ def poisson(rho, gidx, nfft, h, trunc, vcoul=None):
         box[iz, iy, ix] = rho[:, k]
         f = np.fft.rfftn(box)
         f *= vcoul
-        box = np.fft.irfftn(f, s=(n3, n2, n1))
+        box = np.fft.irfftn(f, s=(n3, n2, n1), axes=(0, 1, 2))
         out[:, k] = box[iz, iy, ix]
     return out[:, 0] if single else out
 
