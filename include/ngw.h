@@ -39,7 +39,10 @@ extern "C" {
 
 #define NGW_PREC_F64 0   /* all GEMMs FP64 (the parity path) */
 #define NGW_PREC_F32 1   /* C x, M t and C^T w GEMMs in FP32, FP32 accumulate */
-#define NGW_PREC_BF16 2  /* C x and C^T w in BF16 with per-band Psi scaling, FP32 accumulate */
+#define NGW_PREC_BF16 2  /* C x and C^T w in BF16 with per-band Psi scaling, FP32 accumulate;
+                            M t in FP32 */
+#define NGW_PREC_BF16X3 3 /* as BF16 with each operand split into BF16 head + tail, three
+                             MFMAs per product (about 16 mantissa bits) */
 
 typedef struct ngw_context ngw_context;
 
@@ -66,7 +69,11 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
                  int ld_psi, const double* mmtrx, int ld_m, const double* e1);
 
 /* GEMM precision of the operator (NGW_PREC_*); the Lanczos alpha/beta, the
- * continued fraction, Sigma_c assembly and the QP Newton solve stay FP64. */
+ * continued fraction, Sigma_c assembly and the QP Newton solve stay FP64.
+ * In the low-precision modes p(H) is evaluated as the same polynomial
+ * re-expressed in Chebyshev form on [e_min, e_max] of the driver's fit
+ * (calculate_sigma_lanczos_BLOCK.F90z:166-171) with the Clenshaw recurrence,
+ * which does not amplify rounding the way monomial Horner does. */
 int ngw_set_precision(ngw_context* ctx, int precision);
 
 /*
@@ -196,6 +203,14 @@ int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, l
 int ngw_profile_enable(ngw_context* ctx, int on);
 int ngw_profile_reset(ngw_context* ctx);
 int ngw_profile_read(ngw_context* ctx, int kernel_class, double* total_ms, long* launches);
+
+/*
+ * C = A B through the low-precision MFMA core (kind 0 FP32, 1 BF16,
+ * 2 BF16x3), FP64 device arrays in and out; exported for the library's own
+ * tests of the matrix-core lane layouts.
+ */
+int ngw_gemm_lowp_dev(ngw_context* ctx, int kind, int m, int n, int k, const double* d_a, long lda,
+                      const double* d_b, long ldb, double* d_c, long ldc);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,8 @@ _SIGS = {
     "ngw_profile_enable": ([_vp, C.c_int], C.c_int),
     "ngw_profile_reset": ([_vp], C.c_int),
     "ngw_profile_read": ([_vp, C.c_int, _dp, C.POINTER(C.c_long)], C.c_int),
+    "ngw_gemm_lowp_dev": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_long, _vp, C.c_long, _vp,
+                           C.c_long], C.c_int),
     "ngw_gemm_f64_dev": ([_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_long, _vp, C.c_long, _vp, C.c_long],
                          C.c_int),
 }

@@ -60,5 +60,37 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     return LIB
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DROPIN_CHECK = os.path.join(FORTRAN_DIR, "_bin", "ngw_dropin_check")
+
+
+def build_fortran(verbose: bool = False):
+    """Compile the iso_c_binding module, the NanoGW drop-in routines (against
+    the stand-in types) and the check program with amdflang; None if the
+    Fortran compiler is absent."""
+    fc = os.environ.get("NGW_FC", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        sys.stderr.write("amdflang not found: Fortran drop-in not built\n")
+        return None
+    out = os.path.join(FORTRAN_DIR, "_bin")
+    os.makedirs(out, exist_ok=True)
+    steps = [
+        ["-c", "ngw_c.f90", "-o", os.path.join(out, "ngw_c.o")],
+        ["-c", "ngw_dropin_selftest_types.f90", "-o", os.path.join(out, "types.o")],
+        ["-cpp", "-DNGW_DROPIN_SELFTEST", "-c", "ngw_dropin.F90", "-o", os.path.join(out, "dropin.o")],
+        ["-c", "ngw_dropin_check.f90", "-o", os.path.join(out, "check.o")],
+    ]
+    for st in steps:
+        cmd = [fc, "-O2", f"-J{out}", *st]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, cwd=FORTRAN_DIR)
+    tmp = DROPIN_CHECK + ".tmp"
+    subprocess.run([fc, "-o", tmp, *(os.path.join(out, o) for o in ("check.o", "dropin.o", "types.o", "ngw_c.o")),
+                    f"-L{HERE}", "-lngw", "-Wl,-rpath,$ORIGIN/../.."], check=True, cwd=FORTRAN_DIR)
+    os.replace(tmp, DROPIN_CHECK)
+    return DROPIN_CHECK
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -275,6 +275,7 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
         std::memcpy(&tmp[(size_t)j * n_intp], mmtrx + (size_t)j * ld_m, sizeof(double) * n_intp);
     NGW_TRY(upload(ctx, ctx->mmat, tmp.data(), sizeof(double) * (size_t)n_intp * n_intp));
     NGW_TRY(upload(ctx, ctx->e1, e1, sizeof(double) * nband));
+    ctx->e1_host.assign(e1, e1 + nband);
     // sqrtR(icv) = (e_c - e_v)^0.5 (calculate_sigma_lanczos_BLOCK.F90z:204),
     // sqrtR^3 and sqrtR^2 for the diagonal terms (apply_matvec_BLOCK.F90:198,
     // apply_matvec_lowcomsym.F90:399).
@@ -297,7 +298,7 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
 int ngw_set_precision(ngw_context* ctx, int precision)
 {
     if (!ctx) return fail(NGW_ERR_ARG, "null context");
-    if (precision < NGW_PREC_F64 || precision > NGW_PREC_BF16) return fail(NGW_ERR_ARG, "bad precision");
+    if (precision < NGW_PREC_F64 || precision > NGW_PREC_BF16X3) return fail(NGW_ERR_ARG, "bad precision");
     ctx->precision = precision;
     if (precision != NGW_PREC_F64 && ctx->n_intp > 0) {
         NGW_HIP(hipSetDevice(ctx->device));
@@ -349,21 +350,12 @@ int ngw_polynomial_matvec_isdf_block(ngw_context* ctx, const double* vec, double
     NGW_TRY(ctx->w1.ensure(bytes));
     NGW_TRY(ctx->w2.ensure(bytes));
     NGW_TRY(ctx->w3.ensure(bytes));
+    NGW_TRY(ctx->w4.ensure(bytes));
     double* q = ctx->w0.as<double>();
+    double* out = ctx->w1.as<double>();
     NGW_HIP(hipMemcpyAsync(q, vec, bytes, hipMemcpyHostToDevice, ctx->stream));
-    double* bufs[2] = {ctx->w2.as<double>(), ctx->w3.as<double>()};
-    const double* in = q;
-    double* out = nullptr;
-    for (int k = 1; k <= npoly; ++k) {
-        OperatorArgs a;
-        a.x = in; a.ldx = ncv; a.nvec = blksz; a.tda = tamm_d ? 1 : 0;
-        a.alpha = (k == 1) ? polynomial[0] : 1.0;
-        a.v0 = q; a.ldv0 = ncv; a.gamma = polynomial[k];
-        out = (k == npoly) ? ctx->w1.as<double>() : bufs[k & 1];
-        a.out = out; a.ldo = ncv;
-        NGW_TRY(operator_apply(ctx, a));
-        in = out;
-    }
+    NGW_TRY(poly_apply(ctx, q, out, ctx->w2.as<double>(), ctx->w3.as<double>(), ctx->w4.as<double>(), ncv, blksz,
+                       polynomial, npoly, tamm_d ? 1 : 0, false));
     NGW_HIP(hipMemcpyAsync(hvec, out, bytes, hipMemcpyDeviceToHost, ctx->stream));
     NGW_HIP(hipStreamSynchronize(ctx->stream));
     return NGW_OK;
@@ -386,6 +378,7 @@ int ngw_lanczos_spectra_isdf_block(ngw_context* ctx, const double* v0, const dou
     NGW_TRY(ctx->w1.ensure(vb));
     NGW_TRY(ctx->w2.ensure(vb));
     NGW_TRY(ctx->w3.ensure(vb));
+    NGW_TRY(ctx->w4.ensure(vb));
     NGW_TRY(ctx->ab.ensure(sizeof(double) * 2 * (size_t)niter * blksz));
     NGW_TRY(ctx->zz.ensure(sizeof(double) * 2 * (size_t)std::max(nz, 1) * blksz));
     NGW_TRY(ctx->spec.ensure(sizeof(double) * 2 * (size_t)std::max(nz, 1) * blksz));
@@ -401,8 +394,8 @@ int ngw_lanczos_spectra_isdf_block(ngw_context* ctx, const double* v0, const dou
                                hipMemcpyHostToDevice, s));
     // vec = v0 / v0_norm  (lanczos_spectra_BLOCK.F90:40-42)
     NGW_TRY(scale_columns(s, q, q, ncv, ncv, blksz, ctx->misc.as<double>()));
-    NGW_TRY(lanczos_run(ctx, q, w, ctx->w2.as<double>(), ctx->w3.as<double>(), ncv, blksz, niter,
-                        polynomial, npoly, tamm_d ? 1 : 0, da, db));
+    NGW_TRY(lanczos_run(ctx, q, w, ctx->w2.as<double>(), ctx->w3.as<double>(), ctx->w4.as<double>(), ncv, blksz,
+                        niter, polynomial, npoly, tamm_d ? 1 : 0, da, db));
     if (nz > 0) {
         NGW_TRY(continued_fraction(s, da, db, niter, blksz, ctx->zz.as<double>(), nz,
                                    ctx->spec.as<double>()));
@@ -436,7 +429,7 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
     if (max_cols <= 0) {
         size_t freeb = 0, totalb = 0;
         NGW_HIP(hipMemGetInfo(&freeb, &totalb));
-        const double per_col = 8.0 * (4.0 * ncv + (k1_slots(ctx->nval) + 1.0) * ctx->n_intp);
+        const double per_col = 8.0 * (5.0 * ncv + (k1_slots(ctx->nval) + 2.0) * ctx->n_intp);
         long fit = (long)(0.4 * (double)freeb / per_col);
         cols = std::max(1L, std::min(total, fit));
     }
@@ -453,6 +446,7 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
     NGW_TRY(ctx->w1.ensure(vb));
     NGW_TRY(ctx->w2.ensure(vb));
     NGW_TRY(ctx->w3.ensure(vb));
+    NGW_TRY(ctx->w4.ensure(vb));
     NGW_TRY(ctx->ab.ensure(sizeof(double) * 2 * (size_t)niter * cols));
     NGW_TRY(ctx->zz.ensure(sizeof(double) * 2 * (size_t)nz * total));
     NGW_TRY(ctx->spec.ensure(sizeof(double) * 2 * (size_t)nz * total));
@@ -477,8 +471,8 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
         NGW_TRY(pvectors_f64(ctx, dmb + c0, dnb + c0, nc, tamm_d ? 1 : 0, q, ncv));
         NGW_TRY(column_norms(s, q, ncv, ncv, nc, pn2 + c0, pn + c0));
         NGW_TRY(scale_columns(s, q, q, ncv, ncv, nc, pn + c0));
-        NGW_TRY(lanczos_run(ctx, q, w, ctx->w2.as<double>(), ctx->w3.as<double>(), ncv, nc, niter,
-                            polynomial, npoly, tamm_d ? 1 : 0, da, db));
+        NGW_TRY(lanczos_run(ctx, q, w, ctx->w2.as<double>(), ctx->w3.as<double>(), ctx->w4.as<double>(), ncv, nc,
+                            niter, polynomial, npoly, tamm_d ? 1 : 0, da, db));
         ProfScope ps(ctx, KC_CFRAC);
         NGW_TRY(continued_fraction(s, da, db, niter, nc, zzd + 2 * (size_t)nz * c0, nz,
                                    spec + 2 * (size_t)nz * c0));
@@ -583,6 +577,17 @@ int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, l
     if (m < 0 || n < 0 || k < 0 || lda < m || ldb < k || ldc < m) return fail(NGW_ERR_ARG, "bad GEMM shape");
     NGW_HIP(hipSetDevice(ctx->device));
     NGW_TRY(gemm_f64(ctx->stream, m, n, k, d_a, lda, d_b, ldb, d_c, ldc));
+    NGW_HIP(hipStreamSynchronize(ctx->stream));
+    return NGW_OK;
+}
+
+int ngw_gemm_lowp_dev(ngw_context* ctx, int kind, int m, int n, int k, const double* d_a, long lda,
+                      const double* d_b, long ldb, double* d_c, long ldc)
+{
+    if (!ctx) return fail(NGW_ERR_ARG, "null context");
+    if (m < 0 || n < 0 || k < 0 || lda < m || ldb < k || ldc < m) return fail(NGW_ERR_ARG, "bad GEMM shape");
+    NGW_HIP(hipSetDevice(ctx->device));
+    NGW_TRY(gemm_lowp(ctx->stream, kind, m, n, k, d_a, lda, d_b, ldb, d_c, ldc));
     NGW_HIP(hipStreamSynchronize(ctx->stream));
     return NGW_OK;
 }

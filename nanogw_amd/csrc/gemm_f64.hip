@@ -1,4 +1,4 @@
-// FP64 MFMA GEMM core and the ISDF-factored Casida operator kernels (gfx950).
+// FP64 ISDF-factored Casida operator kernels (gfx950 MFMA).
 //
 // The operator of NanoGW's Lanczos path (shared/apply_matvec_BLOCK.F90:25-208)
 //   y = d o x' + beta s o C^T M C x',  x' = s o x (full) or x (TDA)
@@ -13,115 +13,105 @@
 // (lanczos_spectra_BLOCK.F90:91-95).
 //
 // K1's reduction over v writes per-tile partial sums to a slot buffer instead
-// of using atomics, so every result is bitwise independent of how many other
-// vectors share the launch.
+// of using atomics; a small kernel sums the slots in a fixed order, so every
+// result is bitwise independent of how many other vectors share the launch.
 
-#include "ngw_kernels.h"
+#include "gemm_core_f64.h"
 
 namespace ngw {
 
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-constexpr int BM = 64, BN = 64, BK = 16, NT = 256, PAD = 1;
-
-struct SmemAB {
-    double a[BK][BM + PAD];
-    double b[BK][BN + PAD];
-};
-
-// Loads one BM x BK (or BN x BK) tile through a loader functor into LDS in
-// [k][mn] layout.  kContig loaders are read with consecutive threads along k.
-template <class L, int MN>
-__device__ __forceinline__ void load_tile(const L& ld, int mn0, int k0, int K, double (*dst)[MN + PAD])
-{
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int i = 0; i < (MN * BK) / NT; ++i) {
-        int e = tid + i * NT;
-        int mm, kk;
-        if (L::kContig) {
-            kk = e % BK;
-            mm = e / BK;
-        } else {
-            mm = e % MN;
-            kk = e / MN;
-        }
-        int gm = mn0 + mm, gk = k0 + kk;
-        dst[kk][mm] = (gm < ld.MN && gk < K) ? ld(gm, gk) : 0.0;
-    }
-}
-
-// acc[i][j] holds the 16x16 block (rows wm+16i.., cols wn+16j..) of this wave;
-// lane l, component r <-> row l/16 + 4r, col l%16 (v_mfma_f64_16x16x4f64 output
-// layout, measured on gfx950; A/B operands: lane l holds row/col l%16, k = l/16).
-template <class AL, class BL>
-__device__ __forceinline__ void mainloop(const AL& al, const BL& bl, int m0, int n0, int K,
-                                         f64x4 (&acc)[2][2], SmemAB* sm)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    const int r16 = lane & 15, k4 = lane >> 4;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        load_tile<AL, BM>(al, m0, k0, K, sm->a);
-        load_tile<BL, BN>(bl, n0, k0, K, sm->b);
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            double a0 = sm->a[kk + k4][wm + r16];
-            double a1 = sm->a[kk + k4][wm + 16 + r16];
-            double b0 = sm->b[kk + k4][wn + r16];
-            double b1 = sm->b[kk + k4][wn + 16 + r16];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-}
-
-// Calls f(row, col, value) for the 16 accumulator values this thread holds.
-template <class F>
-__device__ __forceinline__ void for_acc(const f64x4 (&acc)[2][2], F&& f)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                f(wm + 16 * i + (lane >> 4) + 4 * r, wn + 16 * j + (lane & 15), acc[i][j][r]);
-}
-
-// ---------------------------------------------------------------- plain GEMM
-struct ColMajorA {  // A(m, k) = a[m + k*ld], contiguous along m
+// ---------------------------------------------------------------- loaders
+// row(mn) returns a per-row handle; fetch() issues the raw loads of element
+// (mn, k); combine() forms the element from the raw values.
+struct ColMajorA {  // A(m, k) = a[m + k*ld]
     static constexpr bool kContig = false;
+    static constexpr int NRAW = 1;
+    typedef const double* Handle;
     const double* a; long ld; int MN;
-    __device__ double operator()(int m, int k) const { return a[m + (long)k * ld]; }
+    __device__ Handle row(int m) const { return a + m; }
+    __device__ void fetch(Handle h, int k, double (&r)[1]) const { r[0] = h[(long)k * ld]; }
+    __device__ double combine(const double (&r)[1]) const { return r[0]; }
 };
-struct ColMajorB {  // B(k, n) = b[k + n*ld], contiguous along k
+struct ColMajorB {  // B(k, n) = b[k + n*ld]
     static constexpr bool kContig = true;
+    static constexpr int NRAW = 1;
+    typedef const double* Handle;
     const double* b; long ld; int MN;
-    __device__ double operator()(int n, int k) const { return b[k + (long)n * ld]; }
+    __device__ Handle row(int n) const { return b + (long)n * ld; }
+    __device__ void fetch(Handle h, int k, double (&r)[1]) const { r[0] = h[k]; }
+    __device__ double combine(const double (&r)[1]) const { return r[0]; }
+};
+struct ColMajorAT {  // A(m, k) = a[k + m*ld]
+    static constexpr bool kContig = true;
+    static constexpr int NRAW = 1;
+    typedef const double* Handle;
+    const double* a; long ld; int MN;
+    __device__ Handle row(int m) const { return a + (long)m * ld; }
+    __device__ void fetch(Handle h, int k, double (&r)[1]) const { r[0] = h[k]; }
+    __device__ double combine(const double (&r)[1]) const { return r[0]; }
+};
+struct XprimeB {  // x'(c, j = v + nv b) = s(v,c) x_b(v,c); s == null means 1 (TDA)
+    static constexpr bool kContig = false;
+    static constexpr int NRAW = 2;
+    struct Handle { const double* x; const double* s; };
+    const double* x; const double* s; long ldx; int nv; int MN;
+    __device__ Handle row(int j) const
+    {
+        int b = j / nv, v = j - b * nv;
+        return {x + (long)b * ldx + v, s ? s + v : nullptr};
+    }
+    __device__ void fetch(const Handle& h, int c, double (&r)[2]) const
+    {
+        long o = (long)c * nv;
+        r[0] = h.x[o];
+        r[1] = h.s ? h.s[o] : 1.0;
+    }
+    __device__ double combine(const double (&r)[2]) const { return r[1] * r[0]; }
+};
+struct WeightedPsiVB {  // W(mu, j = v + nv b) = PsiV(mu, v) u(mu, b)
+    static constexpr bool kContig = true;
+    static constexpr int NRAW = 2;
+    struct Handle { const double* pv; const double* ub; };
+    const double* psiv; long ldp; const double* u; long ldu; int nv; int MN;
+    __device__ Handle row(int j) const
+    {
+        int b = j / nv, v = j - b * nv;
+        return {psiv + (long)v * ldp, u + (long)b * ldu};
+    }
+    __device__ void fetch(const Handle& h, int mu, double (&r)[2]) const
+    {
+        r[0] = h.pv[mu];
+        r[1] = h.ub[mu];
+    }
+    __device__ double combine(const double (&r)[2]) const { return r[0] * r[1]; }
+};
+struct PairB {  // c_mn(nu) = Psi(nu, m_b) Psi(nu, n_b)  (P-vector build)
+    static constexpr bool kContig = true;
+    static constexpr int NRAW = 2;
+    struct Handle { const double* pm; const double* pn; };
+    const double* psi; long ld; const int* mband; const int* nband; int MN;
+    __device__ Handle row(int b) const { return {psi + (long)mband[b] * ld, psi + (long)nband[b] * ld}; }
+    __device__ void fetch(const Handle& h, int nu, double (&r)[2]) const
+    {
+        r[0] = h.pm[nu];
+        r[1] = h.pn[nu];
+    }
+    __device__ double combine(const double (&r)[2]) const { return r[0] * r[1]; }
 };
 
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(int m, int n, int k, const double* a, long lda,
-                                                      const double* b, long ldb, double* c, long ldc,
-                                                      int mtiles)
+// ---------------------------------------------------------------- plain GEMMs
+template <class AL, class BL>
+__global__ __launch_bounds__(GNT) void gemm_plain_kernel(AL al, BL bl, int m, int n, int k, double alpha,
+                                                         double* c, long ldc, int mtiles, int ntiles)
 {
-    __shared__ SmemAB sm;
-    const int m0 = (blockIdx.x % mtiles) * BM, n0 = (blockIdx.x / mtiles) * BN;
-    f64x4 acc[2][2];
-    mainloop(ColMajorA{a, lda, m}, ColMajorB{b, ldb, n}, m0, n0, k, acc, &sm);
-    for_acc(acc, [&](int row, int col, double val) {
+    __shared__ SmemF64<128, 128> sm;
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int m0 = (tile % mtiles) * 128, n0 = (tile / mtiles) * 128;
+    AccF64<128, 128> acc;
+    gemm_f64_tile<128, 128>(al, bl, m0, n0, k, acc, sm);
+    for_acc_f64<128, 128>(acc, [&](int row, int col, double val) {
         int gm = m0 + row, gn = n0 + col;
-        if (gm < m && gn < n) c[gm + (long)gn * ldc] = val;
+        if (gm < m && gn < n) c[gm + (long)gn * ldc] = alpha * val;
     });
 }
 
@@ -129,163 +119,115 @@ int gemm_f64(hipStream_t s, int m, int n, int k, const double* a, long lda, cons
              long ldb, double* c, long ldc)
 {
     if (m <= 0 || n <= 0) return NGW_OK;
-    int mt = (m + BM - 1) / BM, nt = (n + BN - 1) / BN;
-    hipLaunchKernelGGL(gemm_f64_kernel, dim3(mt * nt), dim3(NT), 0, s, m, n, k, a, lda, b, ldb, c,
-                       ldc, mt);
+    int mt = (m + 127) / 128, nt = (n + 127) / 128;
+    hipLaunchKernelGGL((gemm_plain_kernel<ColMajorA, ColMajorB>), dim3(mt * nt), dim3(GNT), 0, s,
+                       ColMajorA{a, lda, m}, ColMajorB{b, ldb, n}, m, n, k, 1.0, c, ldc, mt, mt * nt);
     NGW_HIP(hipGetLastError());
     return NGW_OK;
-}
-
-// C(m, n) = alpha A(k, m)^T B(k, n): A read along k (zeta^T V zeta of the V build)
-struct ColMajorAT {
-    static constexpr bool kContig = true;
-    const double* a; long ld; int MN;
-    __device__ double operator()(int m, int k) const { return a[k + (long)m * ld]; }
-};
-
-__global__ __launch_bounds__(NT) void gemm_f64_tn_kernel(int m, int n, int k, double alpha, const double* a,
-                                                         long lda, const double* b, long ldb, double* c,
-                                                         long ldc, int mtiles)
-{
-    __shared__ SmemAB sm;
-    const int m0 = (blockIdx.x % mtiles) * BM, n0 = (blockIdx.x / mtiles) * BN;
-    f64x4 acc[2][2];
-    mainloop(ColMajorAT{a, lda, m}, ColMajorB{b, ldb, n}, m0, n0, k, acc, &sm);
-    for_acc(acc, [&](int row, int col, double val) {
-        int gm = m0 + row, gn = n0 + col;
-        if (gm < m && gn < n) c[gm + (long)gn * ldc] = alpha * val;
-    });
 }
 
 int gemm_f64_tn(hipStream_t s, int m, int n, int k, double alpha, const double* a, long lda,
                 const double* b, long ldb, double* c, long ldc)
 {
     if (m <= 0 || n <= 0) return NGW_OK;
-    int mt = (m + BM - 1) / BM, nt = (n + BN - 1) / BN;
-    hipLaunchKernelGGL(gemm_f64_tn_kernel, dim3(mt * nt), dim3(NT), 0, s, m, n, k, alpha, a, lda, b, ldb,
-                       c, ldc, mt);
+    int mt = (m + 127) / 128, nt = (n + 127) / 128;
+    hipLaunchKernelGGL((gemm_plain_kernel<ColMajorAT, ColMajorB>), dim3(mt * nt), dim3(GNT), 0, s,
+                       ColMajorAT{a, lda, m}, ColMajorB{b, ldb, n}, m, n, k, alpha, c, ldc, mt, mt * nt);
     NGW_HIP(hipGetLastError());
     return NGW_OK;
 }
 
 // ---------------------------------------------------------------- K1
-struct K1A {  // PsiC(mu, c)
-    static constexpr bool kContig = false;
-    const double* psic; long ld; int MN;
-    __device__ double operator()(int mu, int c) const { return psic[mu + (long)c * ld]; }
-};
-struct K1B {  // x'(c, j = v + nv*b) = s(v,c) x_b(v,c)
-    static constexpr bool kContig = false;
-    const double* x; const double* s; long ldx; int nv; int MN;
-    __device__ double operator()(int j, int c) const
-    {
-        int b = j / nv, v = j - b * nv;
-        long idx = v + (long)c * nv;
-        double xv = x[idx + (long)b * ldx];
-        return s ? s[idx] * xv : xv;
-    }
-};
+constexpr int K1M = 128, K1N = 64;
 
-__global__ __launch_bounds__(NT) void k1_f64_kernel(K1A al, K1B bl, const double* psiv, long ldp,
-                                                    int nmu, int nc, int nv, int nvec, double* tpart,
-                                                    long ldt, int mtiles)
+__global__ __launch_bounds__(GNT) void k1_f64_kernel(ColMajorA al, XprimeB bl, const double* psiv, long ldp,
+                                                     int nmu, int nc, int nv, int nvec, double* tpart,
+                                                     long ldt, int mtiles, int ntiles)
 {
     __shared__ union {
-        SmemAB ab;
-        double red[BM][BN + PAD];
+        SmemF64<K1M, K1N> ab;
+        double red[K1M][K1N + 2];
     } sm;
-    const int tm = blockIdx.x % mtiles, tn = blockIdx.x / mtiles;
-    const int m0 = tm * BM, n0 = tn * BN;
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int tm = tile % mtiles, tn = tile / mtiles;
+    const int m0 = tm * K1M, n0 = tn * K1N;
     const int N = nv * nvec;
-    f64x4 acc[2][2];
-    mainloop(al, bl, m0, n0, nc, acc, &sm.ab);
-    // acc -> LDS, multiplied by PsiV(mu, v(j))
-    for_acc(acc, [&](int row, int col, double val) {
+    AccF64<K1M, K1N> acc;
+    gemm_f64_tile<K1M, K1N>(al, bl, m0, n0, nc, acc, sm.ab);
+    // T(mu, j) PsiV(mu, v(j)) into LDS, then per-(mu, b) sums over the tile's v range
+    for_acc_f64<K1M, K1N>(acc, [&](int row, int col, double val) {
         int gm = m0 + row, gn = n0 + col;
         double p = 0.0;
         if (gm < nmu && gn < N) p = val * psiv[gm + (long)(gn % nv) * ldp];
         sm.red[row][col] = p;
     });
     __syncthreads();
-    const int nend = min(n0 + BN, N);
+    const int nend = min(n0 + K1N, N);
     const int b_lo = n0 / nv, b_hi = (nend - 1) / nv;
     const int nbt = b_hi - b_lo + 1;
-    for (int e = threadIdx.x; e < BM * nbt; e += NT) {
-        int row = e % BM, b = b_lo + e / BM;
+    for (int e = threadIdx.x; e < K1M * nbt; e += GNT) {
+        int row = e % K1M, b = b_lo + e / K1M;
         int c0 = max(n0, b * nv) - n0, c1 = min(nend, (b + 1) * nv) - n0;
         double s = 0.0;
         for (int cc = c0; cc < c1; ++cc) s += sm.red[row][cc];
         int gm = m0 + row;
-        int slot = tn - (b * nv) / BN;
+        int slot = tn - (b * nv) / K1N;
         if (gm < nmu) tpart[((long)slot * nvec + b) * ldt + gm] = s;
     }
 }
 
-// ---------------------------------------------------------------- K2
-struct K2A {  // M(mu, nu)
-    static constexpr bool kContig = false;
-    const double* m; long ld; int MN;
-    __device__ double operator()(int mu, int nu) const { return m[mu + (long)nu * ld]; }
-};
-struct K2BPart {  // t(nu, b) = sum over the K1 slots that cover vector b
-    static constexpr bool kContig = true;
-    const double* tpart; long ldt; int nvec; int nv; int MN;
-    __device__ double operator()(int b, int nu) const
-    {
-        int first = (b * nv) / BN, last = ((b + 1) * nv - 1) / BN;
-        double s = 0.0;
-        for (int sl = 0; sl <= last - first; ++sl) s += tpart[((long)sl * nvec + b) * ldt + nu];
-        return s;
-    }
-};
-struct K2BPair {  // c_mn(nu) = Psi(nu, m_b) Psi(nu, n_b)  (P-vector build)
-    static constexpr bool kContig = true;
-    const double* psi; long ld; const int* mband; const int* nband; int MN;
-    __device__ double operator()(int b, int nu) const
-    {
-        return psi[nu + (long)mband[b] * ld] * psi[nu + (long)nband[b] * ld];
-    }
-};
+int k1_slots(int nv) { return (nv - 1) / K1N + 2; }
 
-template <class BL>
-__global__ __launch_bounds__(NT) void k2_f64_kernel(K2A al, BL bl, int nmu, int nvec, double* u,
-                                                    long ldu, int mtiles)
+// t(nu, b) = sum over the K1 tiles that covered vector b, in tile order.
+__global__ void k1_reduce_kernel(const double* tpart, long ldt, int nmu, int nvec, int nv, double* t)
 {
-    __shared__ SmemAB sm;
-    const int m0 = (blockIdx.x % mtiles) * BM, n0 = (blockIdx.x / mtiles) * BN;
-    f64x4 acc[2][2];
-    mainloop(al, bl, m0, n0, nmu, acc, &sm);
-    for_acc(acc, [&](int row, int col, double val) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= (long)nmu * nvec) return;
+    const int b = (int)(i / nmu), nu = (int)(i - (long)b * nmu);
+    const int first = (b * nv) / K1N, last = ((b + 1) * nv - 1) / K1N;
+    double s = 0.0;
+    for (int sl = 0; sl <= last - first; ++sl) s += tpart[((long)sl * nvec + b) * ldt + nu];
+    t[nu + (long)b * nmu] = s;
+}
+
+int k1_reduce(hipStream_t s, const double* tpart, long ldt, int nmu, int nvec, int nv, double* t)
+{
+    long total = (long)nmu * nvec;
+    hipLaunchKernelGGL(k1_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, tpart, ldt, nmu,
+                       nvec, nv, t);
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
+}
+
+// ---------------------------------------------------------------- K2
+template <class BL>
+__global__ __launch_bounds__(GNT) void k2_f64_kernel(ColMajorA al, BL bl, int nmu, int nvec, double* u,
+                                                     long ldu, int mtiles, int ntiles)
+{
+    __shared__ SmemF64<64, 64> sm;
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int m0 = (tile % mtiles) * 64, n0 = (tile / mtiles) * 64;
+    AccF64<64, 64> acc;
+    gemm_f64_tile<64, 64>(al, bl, m0, n0, nmu, acc, sm);
+    for_acc_f64<64, 64>(acc, [&](int row, int col, double val) {
         int gm = m0 + row, gn = n0 + col;
         if (gm < nmu && gn < nvec) u[gm + (long)gn * ldu] = val;
     });
 }
 
 // ---------------------------------------------------------------- K3
-struct K3A {  // PsiC(mu, c) read as A(c, mu), contiguous along mu
-    static constexpr bool kContig = true;
-    const double* psic; long ld; int MN;
-    __device__ double operator()(int c, int mu) const { return psic[mu + (long)c * ld]; }
-};
-struct K3B {  // W(mu, j = v + nv*b) = PsiV(mu, v) u(mu, b)
-    static constexpr bool kContig = true;
-    const double* psiv; long ldp; const double* u; long ldu; int nv; int MN;
-    __device__ double operator()(int j, int mu) const
-    {
-        int b = j / nv, v = j - b * nv;
-        return psiv[mu + (long)v * ldp] * u[mu + (long)b * ldu];
-    }
-};
+constexpr int K3M = 64, K3N = 128;
 
-__global__ __launch_bounds__(NT) void k3_f64_kernel(K3A al, K3B bl, int nmu, int nc, int nv, int nvec,
-                                                    OperatorEpilogue ep, int mtiles)
+__global__ __launch_bounds__(GNT) void k3_f64_kernel(ColMajorAT al, WeightedPsiVB bl, int nmu, int nc, int nv,
+                                                     int nvec, OperatorEpilogue ep, int mtiles, int ntiles)
 {
-    __shared__ SmemAB sm;
-    const int m0 = (blockIdx.x % mtiles) * BM, n0 = (blockIdx.x / mtiles) * BN;
+    __shared__ SmemF64<K3M, K3N> sm;
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int m0 = (tile % mtiles) * K3M, n0 = (tile / mtiles) * K3N;
     const int N = nv * nvec;
-    f64x4 acc[2][2];
-    mainloop(al, bl, m0, n0, nmu, acc, &sm);
-    for_acc(acc, [&](int row, int col, double val) {
+    AccF64<K3M, K3N> acc;
+    gemm_f64_tile<K3M, K3N>(al, bl, m0, n0, nmu, acc, sm);
+    for_acc_f64<K3M, K3N>(acc, [&](int row, int col, double val) {
         int c = m0 + row, j = n0 + col;
         if (c < nc && j < N) {
             int b = j / nv, v = j - b * nv;
@@ -295,46 +237,67 @@ __global__ __launch_bounds__(NT) void k3_f64_kernel(K3A al, K3B bl, int nmu, int
 }
 
 // ---------------------------------------------------------------- launchers
-int k1_slots(int nv) { return (nv - 1) / BN + 2; }
+static int launch_k1(ngw_context* ctx, const double* x, long ldx, const double* sx, int B, double* tpart,
+                     double* t)
+{
+    const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond;
+    const double* psi = ctx->psi.as<double>();
+    const long ldp = ctx->ld_psi;
+    hipStream_t s = ctx->stream;
+    {
+        ProfScope ps(ctx, KC_K1);
+        int mt = (nmu + K1M - 1) / K1M, nt = (nv * B + K1N - 1) / K1N;
+        hipLaunchKernelGGL(k1_f64_kernel, dim3(mt * nt), dim3(GNT), 0, s, ColMajorA{psi + (long)nv * ldp, ldp, nmu},
+                           XprimeB{x, sx, ldx, nv, nv * B}, psi, ldp, nmu, nc, nv, B, tpart, (long)nmu, mt,
+                           mt * nt);
+        NGW_HIP(hipGetLastError());
+    }
+    return k1_reduce(s, tpart, (long)nmu, nmu, B, nv, t);
+}
+
+template <class BL>
+static int launch_k2(ngw_context* ctx, const BL& bl, int B, double* u)
+{
+    const int nmu = ctx->n_intp;
+    int mt = (nmu + 63) / 64, nt = (B + 63) / 64;
+    hipLaunchKernelGGL(k2_f64_kernel<BL>, dim3(mt * nt), dim3(GNT), 0, ctx->stream,
+                       ColMajorA{ctx->mmat.as<double>(), ctx->ld_m, nmu}, bl, nmu, B, u, (long)nmu, mt, mt * nt);
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
+}
+
+static int launch_k3(ngw_context* ctx, const double* u, int B, const OperatorEpilogue& ep)
+{
+    const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond;
+    const double* psi = ctx->psi.as<double>();
+    const long ldp = ctx->ld_psi;
+    int mt = (nc + K3M - 1) / K3M, nt = (nv * B + K3N - 1) / K3N;
+    hipLaunchKernelGGL(k3_f64_kernel, dim3(mt * nt), dim3(GNT), 0, ctx->stream,
+                       ColMajorAT{psi + (long)nv * ldp, ldp, nc}, WeightedPsiVB{psi, ldp, u, (long)nmu, nv, nv * B},
+                       nmu, nc, nv, B, ep, mt, mt * nt);
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
+}
 
 int operator_f64(ngw_context* ctx, const OperatorArgs& a)
 {
-    const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond, B = a.nvec;
+    const int nmu = ctx->n_intp, nv = ctx->nval, B = a.nvec;
     if (B <= 0) return NGW_OK;
-    const double* psi = ctx->psi.as<double>();
-    const long ldp = ctx->ld_psi;
-    const double* psiv = psi;
-    const double* psic = psi + (long)nv * ldp;
     const int S = k1_slots(nv);
-    NGW_TRY(ctx->tpart.ensure(sizeof(double) * (size_t)S * B * nmu));
+    NGW_TRY(ctx->tpart.ensure(sizeof(double) * ((size_t)S + 1) * B * nmu));
     NGW_TRY(ctx->ubuf.ensure(sizeof(double) * (size_t)B * nmu));
     double* tpart = ctx->tpart.as<double>();
+    double* t = tpart + (size_t)S * B * nmu;
     double* u = ctx->ubuf.as<double>();
-    hipStream_t s = ctx->stream;
     const double* sx = a.tda ? nullptr : ctx->sqrtr.as<double>();
-    {
-        ProfScope ps(ctx, KC_K1);
-        int mt = (nmu + BM - 1) / BM, nt = (nv * B + BN - 1) / BN;
-        hipLaunchKernelGGL(k1_f64_kernel, dim3(mt * nt), dim3(NT), 0, s,
-                           K1A{psic, ldp, nmu}, K1B{a.x, sx, a.ldx, nv, nv * B}, psiv, ldp, nmu, nc,
-                           nv, B, tpart, (long)nmu, mt);
-        NGW_HIP(hipGetLastError());
-    }
+    NGW_TRY(launch_k1(ctx, a.x, a.ldx, sx, B, tpart, t));
     {
         ProfScope ps(ctx, KC_K2);
-        int mt = (nmu + BM - 1) / BM, nt = (B + BN - 1) / BN;
-        hipLaunchKernelGGL(k2_f64_kernel<K2BPart>, dim3(mt * nt), dim3(NT), 0, s,
-                           K2A{ctx->mmat.as<double>(), ctx->ld_m, nmu},
-                           K2BPart{tpart, (long)nmu, B, nv, B}, nmu, B, u, (long)nmu, mt);
-        NGW_HIP(hipGetLastError());
+        NGW_TRY(launch_k2(ctx, ColMajorB{t, (long)nmu, B}, B, u));
     }
     {
         ProfScope ps(ctx, KC_K3);
-        OperatorEpilogue ep = a.epilogue(ctx);
-        int mt = (nc + BM - 1) / BM, nt = (nv * B + BN - 1) / BN;
-        hipLaunchKernelGGL(k3_f64_kernel, dim3(mt * nt), dim3(NT), 0, s, K3A{psic, ldp, nc},
-                           K3B{psiv, ldp, u, (long)nmu, nv, nv * B}, nmu, nc, nv, B, ep, mt);
-        NGW_HIP(hipGetLastError());
+        NGW_TRY(launch_k3(ctx, u, B, a.epilogue(ctx)));
     }
     return NGW_OK;
 }
@@ -344,34 +307,18 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a)
 int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int nvec, int tda,
                  double* d_p, long ldp_out)
 {
-    const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond, B = nvec;
-    const double* psi = ctx->psi.as<double>();
-    const long ldp = ctx->ld_psi;
-    NGW_TRY(ctx->ubuf.ensure(sizeof(double) * (size_t)B * nmu));
+    const int nmu = ctx->n_intp;
+    NGW_TRY(ctx->ubuf.ensure(sizeof(double) * (size_t)nvec * nmu));
     double* u = ctx->ubuf.as<double>();
-    hipStream_t s = ctx->stream;
     ProfScope ps(ctx, KC_PVEC);
-    {
-        int mt = (nmu + BM - 1) / BM, nt = (B + BN - 1) / BN;
-        hipLaunchKernelGGL(k2_f64_kernel<K2BPair>, dim3(mt * nt), dim3(NT), 0, s,
-                           K2A{ctx->mmat.as<double>(), ctx->ld_m, nmu},
-                           K2BPair{psi, ldp, d_mband, d_nband, B}, nmu, B, u, (long)nmu, mt);
-        NGW_HIP(hipGetLastError());
-    }
-    {
-        OperatorEpilogue ep{};
-        ep.out = d_p;
-        ep.ldo = ldp_out;
-        ep.s = tda ? nullptr : ctx->sqrtr.as<double>();
-        ep.beta = 1.0;
-        ep.alpha = 1.0;
-        int mt = (nc + BM - 1) / BM, nt = (nv * B + BN - 1) / BN;
-        hipLaunchKernelGGL(k3_f64_kernel, dim3(mt * nt), dim3(NT), 0, s,
-                           K3A{psi + (long)nv * ldp, ldp, nc}, K3B{psi, ldp, u, (long)nmu, nv, nv * B},
-                           nmu, nc, nv, B, ep, mt);
-        NGW_HIP(hipGetLastError());
-    }
-    return NGW_OK;
+    NGW_TRY(launch_k2(ctx, PairB{ctx->psi.as<double>(), ctx->ld_psi, d_mband, d_nband, nvec}, nvec, u));
+    OperatorEpilogue ep{};
+    ep.out = d_p;
+    ep.ldo = ldp_out;
+    ep.s = tda ? nullptr : ctx->sqrtr.as<double>();
+    ep.beta = 1.0;
+    ep.alpha = 1.0;
+    return launch_k3(ctx, u, nvec, ep);
 }
 
 }  // namespace ngw

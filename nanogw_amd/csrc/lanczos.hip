@@ -3,6 +3,9 @@
 
 #include "ngw_kernels.h"
 
+#include <cmath>
+#include <vector>
+
 namespace ngw {
 
 constexpr int RT = 256;  // threads of the per-column reduction kernels
@@ -98,39 +101,124 @@ int column_norms(hipStream_t s, const double* v, long ld, long n, int nvec, doub
     return NGW_OK;
 }
 
-// h = p(H) q by Horner (apply_matvec_BLOCK.F90:250-265); the last stage
-// writes w (first Lanczos step) or adds into w (lanczos_spectra_BLOCK.F90:95).
-static int horner(ngw_context* ctx, const double* q, double* w, double* h0, double* h1, long ld,
-                  int nvec, const double* poly, int npoly, int tda, bool accumulate)
+// Chebyshev coefficients c_0..c_n of the monomial polynomial
+// poly = (a_n, ..., a_0) on [lo, hi]: p(x) = sum_k c_k T_k(t),
+// x = ((hi - lo) t + (hi + lo)) / 2.  Horner in the Chebyshev basis in long
+// double (the monomial terms cancel heavily, the result does not).
+static std::vector<double> chebyshev_from_monomial(const double* poly, int npoly, double lo, double hi)
 {
-    const double* in = q;
-    double* bufs[2] = {h0, h1};
+    typedef long double ld;
+    const ld al = ((ld)hi - (ld)lo) / 2, be = ((ld)hi + (ld)lo) / 2;
+    std::vector<ld> c(1, (ld)poly[0]);
     for (int k = 1; k <= npoly; ++k) {
+        // c <- (al t + be) c + a_{n-k}
+        std::vector<ld> r(c.size() + 1, 0.0L);
+        for (size_t j = 0; j < c.size(); ++j) {
+            r[j] += be * c[j];
+            if (j == 0) {
+                r[1] += al * c[0];
+            } else {
+                r[j + 1] += al * c[j] / 2;
+                r[j - 1] += al * c[j] / 2;
+            }
+        }
+        r[0] += (ld)poly[k];
+        c.swap(r);
+    }
+    std::vector<double> out(c.size());
+    for (size_t j = 0; j < c.size(); ++j) out[j] = (double)c[j];
+    return out;
+}
+
+// Spectral interval of the driver's sqrt fit, calculate_sigma_lanczos_BLOCK.F90z:166-171:
+// e_min = (e_{nv+1} - e_nv)^2 / 1.25, e_max = (e_nb - e_1)^2 * 1.25 (Ry^2).
+static void fit_interval(const ngw_context* ctx, double& lo, double& hi)
+{
+    lo = std::pow(ctx->e1_host[ctx->nval] - ctx->e1_host[ctx->nval - 1], 2.0) / 1.25;
+    hi = std::pow(ctx->e1_host[ctx->nband - 1] - ctx->e1_host[0], 2.0) * 1.25;
+    if (!(hi > lo)) {
+        lo = 0.0;
+        hi = 1.0;
+    }
+}
+
+int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, double* h1, double* h2, long ld,
+               int nvec, const double* poly, int npoly, int tda, bool accumulate)
+{
+    if (ctx->precision == PREC_F64 || npoly == 1) {
+        // Horner, apply_matvec_BLOCK.F90:250-265: h = a_n H q + a_{n-1} q; h = H h + a_k q
+        const double* in = q;
+        double* bufs[2] = {h0, h1};
+        for (int k = 1; k <= npoly; ++k) {
+            OperatorArgs a;
+            a.x = in; a.ldx = ld; a.nvec = nvec; a.tda = tda;
+            a.alpha = (k == 1) ? poly[0] : 1.0;
+            a.v0 = q; a.ldv0 = ld; a.gamma = poly[k];
+            const bool last = (k == npoly);
+            double* o = last ? out : bufs[k & 1];
+            a.out = o; a.ldo = ld;
+            if (last && accumulate) { a.acc = out; a.ldacc = ld; }
+            NGW_TRY(operator_apply(ctx, a));
+            in = o;
+        }
+        return NGW_OK;
+    }
+    // Clenshaw on p(x) = sum c_k T_k(t), t = (x - be)/al:
+    //   b_{n+1} = b_{n+2} = 0, b_k = 2 t(b_{k+1}) - b_{k+2} + c_k q, p = t(b_1) - b_2 + c_0 q.
+    // b_n = c_n q is kept implicit (a multiple of q).
+    double lo, hi;
+    fit_interval(ctx, lo, hi);
+    const std::vector<double> c = chebyshev_from_monomial(poly, npoly, lo, hi);
+    const double al = (hi - lo) / 2, be = (hi + lo) / 2;
+    struct V { const double* p; double qc; };  // p, or qc * q when p == null
+    V bk1{nullptr, c[npoly]}, bk2{nullptr, 0.0};
+    double* bufs[3] = {h0, h1, h2};
+    int nb = 0;
+    for (int k = npoly - 1; k >= 0; --k) {
+        const bool last = (k == 0);
+        const double f = last ? 1.0 : 2.0;  // t(b) weight: final stage uses t, the others 2t
         OperatorArgs a;
-        a.x = in; a.ldx = ld; a.nvec = nvec; a.tda = tda;
-        a.alpha = (k == 1) ? poly[0] : 1.0;
-        a.v0 = q; a.ldv0 = ld; a.gamma = poly[k];
-        const bool last = (k == npoly);
-        double* out = last ? w : bufs[k & 1];
-        a.out = out; a.ldo = ld;
-        if (last && accumulate) { a.acc = w; a.ldacc = ld; }
+        a.nvec = nvec; a.tda = tda; a.ldx = ld; a.ldv0 = ld; a.v0 = q;
+        double gam = c[k];
+        if (bk1.p) {
+            a.x = bk1.p; a.alpha = f / al; a.bx = -f * be / al;
+        } else {
+            a.x = q; a.alpha = f * bk1.qc / al; gam += -f * be / al * bk1.qc;
+        }
+        if (bk2.p) {
+            a.y = bk2.p; a.ldy = ld; a.by = -1.0;
+        } else {
+            gam += -bk2.qc;
+        }
+        a.gamma = gam;
+        double* o;
+        if (last) {
+            o = out;
+            if (accumulate) { a.acc = out; a.ldacc = ld; }
+        } else {
+            // a buffer that is neither b_{k+1} nor b_{k+2}
+            o = bufs[nb];
+            nb = (nb + 1) % 3;
+            if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
+            if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
+        }
+        a.out = o; a.ldo = ld;
         NGW_TRY(operator_apply(ctx, a));
-        in = out;
+        bk2 = bk1;
+        bk1 = V{o, 0.0};
     }
     return NGW_OK;
 }
 
-int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double* d_h1, long ld,
-                int nvec, int niter, const double* poly, int npoly, int tda, double* d_a,
-                double* d_b)
+int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double* d_h1, double* d_h2, long ld,
+                int nvec, int niter, const double* poly, int npoly, int tda, double* d_a, double* d_b)
 {
     hipStream_t s = ctx->stream;
     const long n = ctx->ncv;
     for (int k = 1; k <= niter; ++k) {
-        NGW_TRY(horner(ctx, d_q, d_w, d_h0, d_h1, ld, nvec, poly, npoly, tda, k > 1));
+        NGW_TRY(poly_apply(ctx, d_q, d_w, d_h0, d_h1, d_h2, ld, nvec, poly, npoly, tda, k > 1));
         ProfScope ps(ctx, KC_LANCZOS);
-        hipLaunchKernelGGL(lanczos_ab_kernel, dim3(nvec), dim3(RT), 0, s, d_q, d_w, ld, n, d_a, d_b,
-                           niter, k);
+        hipLaunchKernelGGL(lanczos_ab_kernel, dim3(nvec), dim3(RT), 0, s, d_q, d_w, ld, n, d_a, d_b, niter, k);
         NGW_HIP(hipGetLastError());
     }
     return NGW_OK;

@@ -43,7 +43,7 @@ struct DevBuf {
 enum KernelClass { KC_K1 = 0, KC_K2, KC_K3, KC_LANCZOS, KC_CFRAC, KC_PVEC, KC_SIGMA, KC_COUNT };
 
 // Precision of the three GEMM stages of the operator.
-enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2 };
+enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2, PREC_BF16X3 = 3 };
 
 }  // namespace ngw
 
@@ -61,6 +61,7 @@ struct ngw_context {
     ngw::DevBuf mmat;     // n_intp x n_intp, ld = ld_m
     int ld_m = 0;
     ngw::DevBuf e1;       // nband energies (Ry)
+    std::vector<double> e1_host;
     ngw::DevBuf sqrtr;    // ncv: (e_c - e_v)^1/2
     ngw::DevBuf diag_full;  // ncv: sqrtR^3   (applied to sqrtR*x)
     ngw::DevBuf diag_tda;   // ncv: sqrtR^2   (applied to x)

@@ -8,7 +8,8 @@ namespace ngw {
 // Epilogue of the operator's last GEMM (K3), applied per output element
 // (pair index idx of vector b, GEMM value Y = (C^T M C x')(idx)):
 //   hx  = d(idx) x'(idx) + beta s(idx) Y        (x' = s o x, or x if s == null)
-//   r   = alpha hx + gamma v0(idx)               (one Horner stage)
+//   r   = alpha hx + bx x(idx) + by y(idx) + gamma v0(idx)
+//                                                (one Horner or Clenshaw stage)
 //   out = acc(idx) + r                           (Lanczos w += p(H) q)
 struct OperatorEpilogue {
     const double* x = nullptr; long ldx = 0;
@@ -18,17 +19,22 @@ struct OperatorEpilogue {
     const double* v0 = nullptr; long ldv0 = 0; double gamma = 0.0;
     const double* acc = nullptr; long ldacc = 0;
     double* out = nullptr; long ldo = 0;
+    double bx = 0.0;
+    const double* y = nullptr; long ldy = 0; double by = 0.0;
 
     __device__ __forceinline__ void apply(long idx, int b, double Y) const
     {
         double sv = s ? s[idx] : 1.0;
         double hx = (beta * sv) * Y;
+        double xv = 0.0;
         if (x) {
-            double xv = x[idx + (long)b * ldx];
+            xv = x[idx + (long)b * ldx];
             double xp = s ? sv * xv : xv;
             hx = d[idx] * xp + hx;
         }
         double r = alpha * hx;
+        if (bx != 0.0) r = r + bx * xv;
+        if (y) r = r + by * y[idx + (long)b * ldy];
         if (v0) r = r + gamma * v0[idx + (long)b * ldv0];
         if (acc) r = acc[idx + (long)b * ldacc] + r;
         out[idx + (long)b * ldo] = r;
@@ -45,6 +51,8 @@ struct OperatorArgs {
     double alpha = 1.0;
     const double* v0 = nullptr; long ldv0 = 0; double gamma = 0.0;
     const double* acc = nullptr; long ldacc = 0;
+    double bx = 0.0;
+    const double* y = nullptr; long ldy = 0; double by = 0.0;
 
     OperatorEpilogue epilogue(const ngw_context* ctx) const
     {
@@ -57,6 +65,7 @@ struct OperatorArgs {
         e.v0 = v0; e.ldv0 = ldv0; e.gamma = gamma;
         e.acc = acc; e.ldacc = ldacc;
         e.out = out; e.ldo = ldo;
+        e.bx = bx; e.y = y; e.ldy = ldy; e.by = by;
         return e;
     }
 };
@@ -66,6 +75,14 @@ int gemm_f64(hipStream_t s, int m, int n, int k, const double* a, long lda, cons
 int gemm_f64_tn(hipStream_t s, int m, int n, int k, double alpha, const double* a, long lda,
                 const double* b, long ldb, double* c, long ldc);
 int k1_slots(int nv);
+int k1_reduce(hipStream_t s, const double* tpart, long ldt, int nmu, int nvec, int nv, double* t);
+int gemm_lowp(hipStream_t s, int kind, int m, int n, int k, const double* a, long lda, const double* b, long ldb,
+              double* c, long ldc);
+// out = p(H) q (+ out when accumulate): Horner in FP64 mode
+// (apply_matvec_BLOCK.F90:250-265), Clenshaw on the Chebyshev form otherwise.
+// h0, h1, h2: workspaces of the same shape as q.
+int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, double* h1, double* h2, long ld,
+               int nvec, const double* poly, int npoly, int tda, bool accumulate);
 int operator_f64(ngw_context* ctx, const OperatorArgs& a);
 int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int nvec, int tda,
                  double* d_p, long ldp_out);
@@ -76,8 +93,8 @@ int lowp_prepare(ngw_context* ctx);
 int operator_lowp(ngw_context* ctx, const OperatorArgs& a);
 
 // Lanczos pieces (lanczos.hip).
-int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double* d_h1, long ld,
-                int nvec, int niter, const double* poly, int npoly, int tda, double* d_a,
+int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double* d_h1, double* d_h2,
+                long ld, int nvec, int niter, const double* poly, int npoly, int tda, double* d_a,
                 double* d_b);
 int scale_columns(hipStream_t s, const double* in, double* out, long ld, long n, int nvec,
                   const double* d_norm);

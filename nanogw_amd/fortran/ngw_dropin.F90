@@ -1,0 +1,167 @@
+! Drop-in replacements of NanoGW's Lanczos hot-path routines on libngw.
+!
+! Link this file into NanoGW instead of shared/apply_matvec_BLOCK.F90,
+! shared/lanczos_spectra_BLOCK.F90, shared/apply_matvec.f90 and
+! shared/lanczos_spectra.f90: the external subroutines keep the reference's
+! names and argument lists (implicit interfaces, arguments by reference), so
+! Zcalculate_sigma_lanczos_BLOCK and Zcalculate_sigma_lanczos call them
+! unchanged.  The operator (Psi at the ISDF points, Mmtrx, energies) is
+! uploaded to the GPU once and kept resident; it is re-uploaded when the
+! caller passes a different array or size.
+!
+! Scope: one MPI rank per GPU holding the whole transition space
+! (w_grp%npes = 1), all nval x ncond pairs in NanoGW's conduction-outer /
+! valence-inner order (shared/setup_g.F90z:154-168); other layouts stop with
+! a message, as NanoGW's die() would.
+!
+! Built inside the NanoGW tree it uses NanoGW's typedefs and mpi_module; with
+! -DNGW_DROPIN_SELFTEST it compiles against ngw_dropin_selftest_types, a
+! minimal module with the same component names, for a compile and call check
+! of this file outside NanoGW.
+#ifdef NGW_DROPIN_SELFTEST
+#define NGW_TYPES use ngw_dropin_selftest_types
+#else
+#define NGW_TYPES use typedefs; use mpi_module
+#endif
+
+module ngw_session
+
+  use iso_c_binding
+  use ngw_c
+  implicit none
+
+  type(c_ptr), save :: ngw_ctx = c_null_ptr
+  type(c_ptr), save :: up_psi = c_null_ptr, up_m = c_null_ptr
+  integer, save :: up_nintp = -1, up_nband = -1, up_nval = -1
+  integer, save :: ngw_device = 0
+  integer, save :: ngw_precision = 0
+
+contains
+
+  ! nval and nband from the pair list; checks NanoGW's all-pairs ordering.
+  subroutine ngw_pairs(cvpair, ldc, ncv, nval, nband)
+    integer, intent(in) :: ldc, ncv
+    integer, intent(in) :: cvpair(ldc, ncv)
+    integer, intent(out) :: nval, nband
+    integer :: icv, iv, ic
+    nval = maxval(cvpair(1, 1:ncv))
+    nband = maxval(cvpair(2, 1:ncv))
+    if (nval * (nband - nval) /= ncv) then
+      write (6, *) 'libngw: pair list is not all nval x ncond pairs', nval, nband, ncv
+      stop 1
+    end if
+    do icv = 1, ncv
+      iv = mod(icv - 1, nval) + 1
+      ic = nval + (icv - 1) / nval + 1
+      if (cvpair(1, icv) /= iv .or. cvpair(2, icv) /= ic) then
+        write (6, *) 'libngw: pair order differs from conduction-outer/valence-inner at', icv
+        stop 1
+      end if
+    end do
+  end subroutine ngw_pairs
+
+  ! Upload the operator when it is not already resident.
+  subroutine ngw_ensure(n_intp, ld_psi, psi, ld_m, mmtrx, nband, nval, e1)
+    integer, intent(in) :: n_intp, ld_psi, ld_m, nband, nval
+    real(c_double), intent(in), target :: psi(ld_psi, *), mmtrx(ld_m, *)
+    real(c_double), intent(in) :: e1(*)
+    if (.not. c_associated(ngw_ctx)) then
+      call ngw_check(ngw_create(ngw_ctx, int(ngw_device, c_int)), 'ngw_create')
+      call ngw_check(ngw_set_precision(ngw_ctx, int(ngw_precision, c_int)), 'ngw_set_precision')
+    end if
+    if (c_associated(up_psi, c_loc(psi(1, 1))) .and. c_associated(up_m, c_loc(mmtrx(1, 1))) .and. &
+        up_nintp == n_intp .and. up_nband == nband .and. up_nval == nval) return
+    call ngw_check(ngw_set_isdf(ngw_ctx, int(n_intp, c_int), int(nband, c_int), int(nval, c_int), &
+                                psi, int(ld_psi, c_int), mmtrx, int(ld_m, c_int), e1), 'ngw_set_isdf')
+    call ngw_check(ngw_set_precision(ngw_ctx, int(ngw_precision, c_int)), 'ngw_set_precision')
+    up_psi = c_loc(psi(1, 1))
+    up_m = c_loc(mmtrx(1, 1))
+    up_nintp = n_intp
+    up_nband = nband
+    up_nval = nval
+  end subroutine ngw_ensure
+
+  subroutine ngw_release()
+    integer(c_int) :: st
+    if (c_associated(ngw_ctx)) st = ngw_destroy(ngw_ctx)
+    ngw_ctx = c_null_ptr
+    up_psi = c_null_ptr
+    up_m = c_null_ptr
+    up_nintp = -1
+  end subroutine ngw_release
+
+end module ngw_session
+
+! shared/apply_matvec_BLOCK.F90:25
+subroutine matvec_isdf_BLOCK(vec, Hvec, ncv_loc, isdf_in, wfn, cvpair, ncv, sqrtR, &
+                             MC_vec, CMC_vec, tmp_vec, tmp_Hvec, tmp_cvpair, blksz)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, blksz, ncv
+  real(dp), intent(in) :: vec(ncv_loc, blksz), sqrtR(ncv_loc)
+  real(dp), intent(out) :: Hvec(ncv_loc, blksz)
+  type(ISDF), intent(in) :: isdf_in
+  type(wavefunction), intent(in) :: wfn
+  integer, intent(in) :: cvpair(4, ncv)
+  real(dp), intent(inout) :: MC_vec(*), CMC_vec(*), tmp_vec(*), tmp_Hvec(*)
+  integer, intent(inout) :: tmp_cvpair(*)
+  integer :: nval, nband
+  if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
+  call ngw_pairs(cvpair, 4, ncv, nval, nband)
+  call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+  call ngw_check(ngw_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), int(blksz, c_int), 0_c_int), &
+                 'matvec_isdf_BLOCK')
+end subroutine matvec_isdf_BLOCK
+
+! shared/apply_matvec_BLOCK.F90:210
+subroutine polynomial_matvec_isdf_BLOCK(vec, Hvec, ncv_loc, polynomial, npoly, isdf_in, wfn, cvpair, ncv, &
+                                        sqrtR, MC_vec, CMC_vec, tmp_vec, tmp_Hvec, tmp_cvpair, blksz)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, npoly, blksz, ncv
+  real(dp), intent(in) :: vec(ncv_loc, blksz), polynomial(npoly + 1), sqrtR(ncv_loc)
+  real(dp), intent(out) :: Hvec(ncv_loc, blksz)
+  type(ISDF), intent(in) :: isdf_in
+  type(wavefunction), intent(in) :: wfn
+  integer, intent(in) :: cvpair(4, ncv)
+  real(dp), intent(inout) :: MC_vec(*), CMC_vec(*), tmp_vec(*), tmp_Hvec(*)
+  integer, intent(inout) :: tmp_cvpair(*)
+  integer :: nval, nband
+  if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
+  call ngw_pairs(cvpair, 4, ncv, nval, nband)
+  call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+  call ngw_check(ngw_polynomial_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), polynomial, &
+                                                  int(npoly, c_int), int(blksz, c_int), 0_c_int), &
+                 'polynomial_matvec_isdf_BLOCK')
+end subroutine polynomial_matvec_isdf_BLOCK
+
+! shared/lanczos_spectra_BLOCK.F90:7
+subroutine lanczos_spectra_isdf_BLOCK(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_in, wfn, cvpair, ncv, sqrtR, &
+                                      polynomial, npoly, spectra, MC_vec, CMC_vec, tmp_vec, tmp_Hvec, &
+                                      tmp_cvpair, blksz)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, niter, nz, npoly, blksz, ncv
+  complex(dp), intent(in) :: zz(nz, blksz)
+  real(dp), intent(in) :: v0(ncv_loc, blksz), polynomial(npoly + 1), v0_norm(blksz), sqrtR(ncv_loc)
+  complex(dp), intent(out) :: spectra(nz, blksz)
+  type(ISDF), intent(in) :: isdf_in
+  type(wavefunction), intent(in) :: wfn
+  integer, intent(in) :: cvpair(4, ncv)
+  real(dp), intent(inout) :: MC_vec(*), CMC_vec(*), tmp_vec(*), tmp_Hvec(*)
+  integer, intent(inout) :: tmp_cvpair(*)
+  integer :: nval, nband
+  if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
+  call ngw_pairs(cvpair, 4, ncv, nval, nband)
+  call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+  call ngw_check(ngw_lanczos_spectra_isdf_block(ngw_ctx, v0, v0_norm, int(ncv_loc, c_int), zz, int(nz, c_int), &
+                                                int(niter, c_int), polynomial, int(npoly, c_int), spectra, &
+                                                int(blksz, c_int), 0_c_int, c_null_ptr, c_null_ptr), &
+                 'lanczos_spectra_isdf_BLOCK')
+end subroutine lanczos_spectra_isdf_BLOCK

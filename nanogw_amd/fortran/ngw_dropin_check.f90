@@ -1,0 +1,46 @@
+! Compile-and-call check of ngw_dropin.F90 outside NanoGW (see
+! ngw_dropin_selftest_types.f90): reads a small problem from a stream file
+! (argument 1), calls the drop-in matvec_isdf_BLOCK and
+! lanczos_spectra_isdf_BLOCK as Zcalculate_sigma_lanczos_BLOCK does, and
+! writes Hvec and the spectra to a stream file (argument 2).
+program ngw_dropin_check
+  use ngw_dropin_selftest_types
+  use ngw_session, only: ngw_release
+  implicit none
+  type(ISDF) :: isdf_in
+  type(wavefunction) :: wfn
+  integer :: nmu, nv, nc, nb, ncv, blksz, nz, niter, npoly, icv, u
+  integer, allocatable :: cvpair(:, :), tmp_cvpair(:)
+  real(dp), allocatable :: vec(:, :), hvec(:, :), sqrtR(:), poly(:), v0norm(:), dummy(:)
+  complex(dp), allocatable :: zz(:, :), spectra(:, :)
+  character(len=512) :: fin, fout
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read (u) nmu, nv, nc, blksz, nz, niter, npoly
+  nb = nv + nc
+  ncv = nv * nc
+  allocate (isdf_in%Psi_intp_loc(nmu, nb, 1, 1), isdf_in%Mmtrx_loc(nmu, nmu, 1, 1, 1, 1, 1), wfn%e1(nb))
+  allocate (vec(ncv, blksz), hvec(ncv, blksz), sqrtR(ncv), poly(npoly + 1), v0norm(blksz))
+  allocate (zz(nz, blksz), spectra(nz, blksz), cvpair(4, ncv), tmp_cvpair(2 * ncv), dummy(1))
+  read (u) isdf_in%Psi_intp_loc, isdf_in%Mmtrx_loc, wfn%e1, vec, poly, zz
+  close (u)
+  isdf_in%n_intp_r = nmu
+  ! NanoGW's pair list: conduction outer, valence inner (setup_g.F90z:154-168)
+  do icv = 1, ncv
+    cvpair(1, icv) = mod(icv - 1, nv) + 1
+    cvpair(2, icv) = nv + (icv - 1) / nv + 1
+    cvpair(3:4, icv) = 1
+    sqrtR(icv) = sqrt(wfn%e1(cvpair(2, icv)) - wfn%e1(cvpair(1, icv)))
+  end do
+  call matvec_isdf_BLOCK(vec, hvec, ncv, isdf_in, wfn, cvpair, ncv, sqrtR, dummy, dummy, dummy, dummy, &
+                         tmp_cvpair, blksz)
+  v0norm = sqrt(sum(vec**2, dim=1))
+  call lanczos_spectra_isdf_BLOCK(vec, v0norm, ncv, zz, nz, niter, isdf_in, wfn, cvpair, ncv, sqrtR, poly, &
+                                  npoly, spectra, dummy, dummy, dummy, dummy, tmp_cvpair, blksz)
+  call ngw_release()
+  open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write (u) hvec, spectra
+  close (u)
+end program ngw_dropin_check

@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 
-PREC = {"f64": 0, "f32": 1, "bf16": 2}
+PREC = {"f64": 0, "f32": 1, "bf16": 2, "bf16x3": 3}
 
 
 def _f64(a):
@@ -185,6 +185,10 @@ class IsdfContext:
     def matvec_dev(self, d_vec: int, d_hvec: int, ld: int, blksz: int, tamm_d=False):
         _lib.check(self._lib.ngw_matvec_isdf_block_dev(self._h, d_vec, d_hvec, int(ld), int(blksz),
                                                        int(tamm_d)), "ngw_matvec_isdf_block_dev")
+
+    def gemm_lowp_dev(self, kind, m, n, k, d_a, lda, d_b, ldb, d_c, ldc):
+        _lib.check(self._lib.ngw_gemm_lowp_dev(self._h, kind, m, n, k, d_a, lda, d_b, ldb, d_c, ldc),
+                   "ngw_gemm_lowp_dev")
 
     def gemm_f64_dev(self, m, n, k, d_a, lda, d_b, ldb, d_c, ldc):
         _lib.check(self._lib.ngw_gemm_f64_dev(self._h, m, n, k, d_a, lda, d_b, ldb, d_c, ldc),

@@ -210,9 +210,12 @@ def isdf_zeta(psi, intp, nv, ridge=0.0, device=None):
             A = A + ridge * torch.trace(A) / A.shape[0] * torch.eye(A.shape[0], dtype=A.dtype, device=device)
         B = (pv @ pi_v.T) * (t @ pi_n.T)
         del t, pv
-        z = torch.linalg.solve(A, B.T).T
-        out = np.asfortranarray(z.cpu().numpy())
-        del A, B, z
+        lu, piv = torch.linalg.lu_factor(A)
+        out = np.empty((B.shape[0], A.shape[0]), order="F")
+        for r0 in range(0, B.shape[0], 8192):
+            zt = torch.linalg.lu_solve(lu, piv, B[r0:r0 + 8192].T.contiguous())
+            out[r0:r0 + zt.shape[1], :] = zt.T.cpu().numpy()
+        del A, B, lu, piv
         torch.cuda.empty_cache()
         return out
     pv = psi[:, :nv]

@@ -1,0 +1,133 @@
+"""Mixed-precision operator (FP32 / BF16 / BF16x3 GEMMs, FP64 recurrence)
+against the FP64 oracle.  Errors are stated per precision; the bars are the
+north-star's: QP energies within 1 meV of the FP64 result for the precisions
+that claim it (FP32, BF16x3), Sigma_c within 1e-4 relative on the QP window.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+RYD = 13.60569193
+
+
+@pytest.fixture(scope="module")
+def wellcond():
+    """Small molecule with a Tikhonov-regularised zeta solve (well-conditioned M,
+    like the Nmu << nv nc regime of the large configurations)."""
+    from nanogw_amd import synth
+
+    return synth.make_molecule(5.0, 0.5, 6, 30, 60, seed=31337, ridge=1e-6)
+
+
+@pytest.mark.parametrize("kind,tol", [(0, 1e-5), (1, 2e-2), (2, 5e-5)])
+def test_gemm_lowp_lane_layout(gpu, ngw_ctx, kind, tol):
+    import torch
+
+    rng = np.random.default_rng(7)
+    for (m, n, k) in [(16, 16, 32), (70, 45, 33), (64, 128, 96), (129, 3, 200)]:
+        a = rng.normal(size=(m, k))
+        b = rng.normal(size=(k, n))
+        ta = torch.tensor(a.T.copy(), dtype=torch.float64, device=gpu)
+        tb = torch.tensor(b.T.copy(), dtype=torch.float64, device=gpu)
+        tc = torch.zeros((n, m), dtype=torch.float64, device=gpu)
+        torch.cuda.synchronize()
+        ngw_ctx.gemm_lowp_dev(kind, m, n, k, ta.data_ptr(), m, tb.data_ptr(), k, tc.data_ptr(), m)
+        c = tc.cpu().numpy().T
+        # relative to the scale of the products
+        err = np.abs(c - a @ b).max() / np.sqrt(k)
+        print("lowp gemm", kind, (m, n, k), err)
+        assert err < tol, (kind, m, n, k, err)
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16x3", 1e-4), ("bf16", 3e-2)])
+def test_matvec_lowp_vs_oracle(ngw_ctx, wellcond, prec, tol):
+    from oracle import oracle as O
+
+    m = wellcond
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    ngw_ctx.set_precision(prec)
+    try:
+        x = np.random.default_rng(1).normal(size=(m.nv * m.nc, 5))
+        y = ngw_ctx.matvec_isdf_BLOCK(x)
+        s = O.sqrt_r(m.e1, m.nv, m.nc)
+        ref = O.matvec_block(m.psi_intp, m.mmtrx, s, x, m.nv, m.nc)
+        err = rel_err(y, ref)
+        print("matvec", prec, err)
+        assert err < tol
+    finally:
+        ngw_ctx.set_precision("f64")
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16x3", 1e-4)])
+def test_clenshaw_polynomial_vs_oracle(ngw_ctx, wellcond, prec, tol):
+    """Low-precision p(H) (Chebyshev/Clenshaw form of the pinned monomial
+    polynomial) against the oracle's FP64 Horner."""
+    from nanogw_amd.lanczos import lanczos_polynomial
+    from oracle import oracle as O
+
+    m = wellcond
+    poly = lanczos_polynomial(m.e1, m.nv, 12)
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    ngw_ctx.set_precision(prec)
+    try:
+        x = np.random.default_rng(2).normal(size=(m.nv * m.nc, 3))
+        y = ngw_ctx.polynomial_matvec_isdf_BLOCK(x, poly)
+        s = O.sqrt_r(m.e1, m.nv, m.nc)
+        ref = O.polynomial_matvec_block(m.psi_intp, m.mmtrx, s, x, m.nv, m.nc, poly)
+        err = rel_err(y, ref)
+        print("p(H)", prec, err)
+        assert err < tol
+    finally:
+        ngw_ctx.set_precision("f64")
+
+
+def test_clenshaw_fp64_equals_horner(ngw_ctx, wellcond):
+    """The Chebyshev re-expression is the same polynomial: with FP32 GEMMs the
+    difference to FP64 Horner is at the FP32 level, far below the monomial
+    cancellation error FP32 Horner would give."""
+    from nanogw_amd.lanczos import lanczos_polynomial
+
+    m = wellcond
+    poly = lanczos_polynomial(m.e1, m.nv, 12)
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    x = np.random.default_rng(3).normal(size=(m.nv * m.nc, 2))
+    y64 = ngw_ctx.polynomial_matvec_isdf_BLOCK(x, poly)
+    ngw_ctx.set_precision("f32")
+    try:
+        y32 = ngw_ctx.polynomial_matvec_isdf_BLOCK(x, poly)
+    finally:
+        ngw_ctx.set_precision("f64")
+    assert rel_err(y32, y64) < 1e-5
+
+
+@pytest.mark.parametrize("prec,qp_tol_ev", [("f32", 1e-3), ("bf16x3", 1e-3), ("bf16", 2e-2)])
+def test_sigma_qp_mixed_vs_fp64(ngw_ctx, wellcond, prec, qp_tol_ev):
+    """Sigma_c and QP energies of the mixed-precision path against the FP64
+    oracle (niter = 60, all bands, 4 QP states)."""
+    from nanogw_amd import synth
+    from nanogw_amd.lanczos import lanczos_polynomial
+    from oracle import oracle as O
+
+    m = wellcond
+    poly = lanczos_polynomial(m.e1, m.nv, 12)
+    qp = synth.qp_states(m.nv, 4)
+    sd = synth.sigma_defaults(m.nv)
+    nen, deltae, ecuts, niter = sd["nen"], sd["deltae"], sd["ecuts"], 60
+    ref = O.sigma_lanczos(m.psi_intp, m.mmtrx, m.e1, m.nv, m.nc, qp, nen, deltae, ecuts, niter, poly, 36)
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    ngw_ctx.set_precision(prec)
+    try:
+        sig = ngw_ctx.calculate_sigma_lanczos(qp, nen, deltae, ecuts, niter, poly)
+    finally:
+        ngw_ctx.set_precision("f64")
+    sigx = synth.exchange_diag(m.psi_intp, m.mmtrx, m.nv, qp)
+    e = m.e1[qp] * RYD
+    got = ngw_ctx.qp_newton(sig * RYD, deltae * RYD, e, e, sigx * RYD, np.zeros(len(qp)))["eqp"]
+    exp = np.array([O.qp_solve(ref[:, i] * RYD, deltae * RYD, e[i], e[i], sigx[i] * RYD, 0.0)["eqp"]
+                    for i in range(len(qp))])
+    dq = np.abs(got - exp).max()
+    print(f"{prec}: max |dE_qp| = {dq * 1e3:.4f} meV, Sigma_c rel err {rel_err(sig, ref):.2e}")
+    assert dq < qp_tol_ev

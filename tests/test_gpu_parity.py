@@ -171,13 +171,24 @@ def test_qp_newton_vs_oracle(ngw_ctx):
 
 
 def test_errors_raise(ngw_ctx, mini):
+    """Size mismatches and bad indices come back as statuses, not faults."""
+    import ctypes as C
+
     from nanogw_amd._lib import NgwError
 
     ngw_ctx.set_isdf(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv)
-    with pytest.raises(NgwError):
-        ngw_ctx.lanczos_spectra_isdf_BLOCK(np.ones((7, 1)), [1.0], np.zeros((3, 1)), 5, [1.0, 0.0])
+    lib = ngw_ctx._lib
+    v = np.ones(7)
+    spec = np.zeros(6)
+    dp = C.POINTER(C.c_double)
+    st = lib.ngw_lanczos_spectra_isdf_block(ngw_ctx.handle, v.ctypes.data_as(dp), v.ctypes.data_as(dp), 7,
+                                            spec.ctypes.data_as(dp), 3, 5, v.ctypes.data_as(dp), 1,
+                                            spec.ctypes.data_as(dp), 1, 0, None, None)
+    assert st == -1 and b"ncv" in lib.ngw_last_error()
     with pytest.raises(NgwError):
         ngw_ctx.calculate_sigma_lanczos([999], 10, 1.0, 0.01, 5, [1.0, 0.0])
+    with pytest.raises(NgwError):
+        ngw_ctx.isdf_coulomb_kernel(mini.zeta, mini.gidx, (4, 4, 4), mini.h, mini.trunc)
 
 
 def test_isdf_coulomb_kernel_vs_oracle(ngw_ctx, mini):
