@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15}
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "bf16x3": 2.5e15}
 HBM_PEAK = 8.0e12
 RYD = 13.60569193
 
@@ -79,16 +79,40 @@ def cpu_baseline(mol, poly, niter, nb_sample=8, target_s=12.0):
                       f"x{mol.nb / nb_sample:.1f} bands per QP solve"}
 
 
+def fp64_sample(ctx, mol, poly, niter, steps, q):
+    """Plain FP64 GPU path on the same batch shape (q states x N_b bands) for
+    `steps` Lanczos steps; every step does identical work, so QP solves/s is
+    extrapolated linearly in the step count."""
+    nv, nc, nb = mol.nv, mol.nc, mol.nb
+    B = q * nb
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(size=(nv * nc, B))
+    vn = np.linalg.norm(v0, axis=0)
+    zz = np.zeros((2, B), np.complex128) + 0.02j
+    ctx.set_precision("f64")
+    ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, 1, poly)  # allocate
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, steps, poly)
+    dt = time.perf_counter() - t
+    per_qp = dt * (niter / steps) / q
+    return {"qp_solves_per_s": 1.0 / per_qp, "timed_lanczos_steps": steps, "of_niter": niter,
+            "seconds": dt, "columns": B,
+            "note": "FP64 GPU path timed on the full batch for a few Lanczos steps, extrapolated"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16", "bf16x3"])
     ap.add_argument("--qp-per-step", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="Lanczos steps of the plain FP64 GPU path timed on the same batch for comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the ISDF-Lanczos path "
@@ -159,6 +183,10 @@ def main():
                 "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
                 "avg_launch_us": avg_s * 1e6, "launches": cnt,
                 "flop_per_launch": flops[dom]}
+    fp64_ref = None
+    if args.precision != "f64" and args.fp64_steps > 0:
+        fp64_ref = fp64_sample(ctx, mol, poly, niter, args.fp64_steps, q)
+        fp64_ref["speedup"] = value / fp64_ref["qp_solves_per_s"]
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle sample)...")
@@ -186,6 +214,7 @@ def main():
         "roofline": roof,
         "kernel_ms": {k: v[0] for k, v in prof.items()},
         "cpu_baseline": cpu,
+        "fp64_gpu": fp64_ref,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

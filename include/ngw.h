@@ -212,6 +212,38 @@ int ngw_profile_read(ngw_context* ctx, int kernel_class, double* total_ms, long*
 int ngw_gemm_lowp_dev(ngw_context* ctx, int kind, int m, int n, int k, const double* d_a, long lda,
                       const double* d_b, long ldb, double* d_c, long ldc);
 
+/*
+ * Batched and continuously served GW (no single NanoGW counterpart: it runs
+ * the per-request work of Zcalculate_sigma_lanczos_BLOCK,
+ * SIGMA/calculate_sigma_lanczos_BLOCK.F90z:201-444, for many molecules at
+ * once).  A batch uses its context's device, stream and precision.
+ *   add_system     upload one molecule's ISDF operator (kept resident)
+ *   request_bytes  HBM a request would allocate (for admission control)
+ *   submit         one GW request: nqp QP states of a system; builds the P
+ *                  vectors and the Lanczos start on the device
+ *   step           advance every unfinished request by nsteps Lanczos steps
+ *                  in grouped launches (use_graph = 1: one hipGraph per step,
+ *                  re-captured when requests join or leave); requests that
+ *                  reach niter get their continued fraction and Sigma_c
+ *   result         Sigma_c(nen, nqp) complex (Ry) once done
+ * A request's result does not depend on the other requests in the batch.
+ */
+typedef struct ngw_batch ngw_batch;
+int ngw_batch_create(ngw_context* ctx, ngw_batch** batch);
+int ngw_batch_destroy(ngw_batch* batch);
+int ngw_batch_add_system(ngw_batch* batch, int n_intp, int nband, int nval, const double* psi_intp,
+                         int ld_psi, const double* mmtrx, int ld_m, const double* e1, int* sys_id);
+int ngw_batch_remove_system(ngw_batch* batch, int sys_id);
+int ngw_batch_request_bytes(ngw_batch* batch, int sys_id, int nqp, int niter, int nen, long* bytes);
+int ngw_batch_submit(ngw_batch* batch, int sys_id, int nqp, const int* qp_band, int nen, double deltae,
+                     double ecuts, int niter, const double* polynomial, int npoly, int tamm_d,
+                     int* req_id);
+int ngw_batch_step(ngw_batch* batch, int nsteps, int use_graph, int* n_finished);
+int ngw_batch_active(ngw_batch* batch, int* n_active);
+int ngw_batch_result(ngw_batch* batch, int req_id, double* sigma_c, int* done);
+int ngw_batch_release(ngw_batch* batch, int req_id);
+int ngw_batch_stats(ngw_batch* batch, long* steps_run, long* graph_captures);
+
 #ifdef __cplusplus
 }
 #endif

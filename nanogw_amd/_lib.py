@@ -46,6 +46,18 @@ _SIGS = {
     "ngw_profile_read": ([_vp, C.c_int, _dp, C.POINTER(C.c_long)], C.c_int),
     "ngw_gemm_lowp_dev": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_long, _vp, C.c_long, _vp,
                            C.c_long], C.c_int),
+    "ngw_batch_create": ([_vp, C.POINTER(_vp)], C.c_int),
+    "ngw_batch_destroy": ([_vp], C.c_int),
+    "ngw_batch_add_system": ([_vp, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _ip], C.c_int),
+    "ngw_batch_remove_system": ([_vp, C.c_int], C.c_int),
+    "ngw_batch_request_bytes": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_long)], C.c_int),
+    "ngw_batch_submit": ([_vp, C.c_int, C.c_int, _ip, C.c_int, C.c_double, C.c_double, C.c_int, _dp, C.c_int,
+                          C.c_int, _ip], C.c_int),
+    "ngw_batch_step": ([_vp, C.c_int, C.c_int, _ip], C.c_int),
+    "ngw_batch_active": ([_vp, _ip], C.c_int),
+    "ngw_batch_result": ([_vp, C.c_int, _dp, _ip], C.c_int),
+    "ngw_batch_release": ([_vp, C.c_int], C.c_int),
+    "ngw_batch_stats": ([_vp, C.POINTER(C.c_long), C.POINTER(C.c_long)], C.c_int),
     "ngw_gemm_f64_dev": ([_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_long, _vp, C.c_long, _vp, C.c_long],
                          C.c_int),
 }

@@ -230,7 +230,7 @@ int ngw_destroy(ngw_context* ctx)
         (void)hipEventDestroy(r.b);
     }
     for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(ctx->stream);
+    if (!ctx->shared_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return NGW_OK;
 }

@@ -142,26 +142,26 @@ static void fit_interval(const ngw_context* ctx, double& lo, double& hi)
     }
 }
 
-int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, double* h1, double* h2, long ld,
-               int nvec, const double* poly, int npoly, int tda, bool accumulate)
+std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const double* q, double* out, double* h0,
+                                 double* h1, double* h2, const double* poly, int npoly, bool accumulate)
 {
-    if (ctx->precision == PREC_F64 || npoly == 1) {
+    std::vector<PolyStage> plan;
+    if (precision == PREC_F64 || npoly == 1) {
         // Horner, apply_matvec_BLOCK.F90:250-265: h = a_n H q + a_{n-1} q; h = H h + a_k q
         const double* in = q;
         double* bufs[2] = {h0, h1};
         for (int k = 1; k <= npoly; ++k) {
-            OperatorArgs a;
-            a.x = in; a.ldx = ld; a.nvec = nvec; a.tda = tda;
-            a.alpha = (k == 1) ? poly[0] : 1.0;
-            a.v0 = q; a.ldv0 = ld; a.gamma = poly[k];
+            PolyStage st;
+            st.x = in;
+            st.alpha = (k == 1) ? poly[0] : 1.0;
+            st.gamma = poly[k];
             const bool last = (k == npoly);
-            double* o = last ? out : bufs[k & 1];
-            a.out = o; a.ldo = ld;
-            if (last && accumulate) { a.acc = out; a.ldacc = ld; }
-            NGW_TRY(operator_apply(ctx, a));
-            in = o;
+            st.out = last ? out : bufs[k & 1];
+            st.accumulate = last && accumulate;
+            plan.push_back(st);
+            in = st.out;
         }
-        return NGW_OK;
+        return plan;
     }
     // Clenshaw on p(x) = sum c_k T_k(t), t = (x - be)/al:
     //   b_{n+1} = b_{n+2} = 0, b_k = 2 t(b_{k+1}) - b_{k+2} + c_k q, p = t(b_1) - b_2 + c_0 q.
@@ -176,25 +176,24 @@ int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, doubl
     int nb = 0;
     for (int k = npoly - 1; k >= 0; --k) {
         const bool last = (k == 0);
-        const double f = last ? 1.0 : 2.0;  // t(b) weight: final stage uses t, the others 2t
-        OperatorArgs a;
-        a.nvec = nvec; a.tda = tda; a.ldx = ld; a.ldv0 = ld; a.v0 = q;
+        const double f = last ? 1.0 : 2.0;  // t(b) weight: the final stage uses t, the others 2t
+        PolyStage st;
         double gam = c[k];
         if (bk1.p) {
-            a.x = bk1.p; a.alpha = f / al; a.bx = -f * be / al;
+            st.x = bk1.p; st.alpha = f / al; st.bx = -f * be / al;
         } else {
-            a.x = q; a.alpha = f * bk1.qc / al; gam += -f * be / al * bk1.qc;
+            st.x = q; st.alpha = f * bk1.qc / al; gam += -f * be / al * bk1.qc;
         }
         if (bk2.p) {
-            a.y = bk2.p; a.ldy = ld; a.by = -1.0;
+            st.y = bk2.p; st.by = -1.0;
         } else {
             gam += -bk2.qc;
         }
-        a.gamma = gam;
+        st.gamma = gam;
         double* o;
         if (last) {
             o = out;
-            if (accumulate) { a.acc = out; a.ldacc = ld; }
+            st.accumulate = accumulate;
         } else {
             // a buffer that is neither b_{k+1} nor b_{k+2}
             o = bufs[nb];
@@ -202,10 +201,27 @@ int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, doubl
             if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
             if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
         }
-        a.out = o; a.ldo = ld;
-        NGW_TRY(operator_apply(ctx, a));
+        st.out = o;
+        plan.push_back(st);
         bk2 = bk1;
         bk1 = V{o, 0.0};
+    }
+    return plan;
+}
+
+int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, double* h1, double* h2, long ld,
+               int nvec, const double* poly, int npoly, int tda, bool accumulate)
+{
+    const std::vector<PolyStage> plan = plan_poly(ctx, ctx->precision, q, out, h0, h1, h2, poly, npoly, accumulate);
+    for (const PolyStage& st : plan) {
+        OperatorArgs a;
+        a.x = st.x; a.ldx = ld; a.nvec = nvec; a.tda = tda;
+        a.alpha = st.alpha; a.bx = st.bx;
+        a.y = st.y; a.ldy = ld; a.by = st.by;
+        a.v0 = q; a.ldv0 = ld; a.gamma = st.gamma;
+        a.out = st.out; a.ldo = ld;
+        if (st.accumulate) { a.acc = st.out; a.ldacc = ld; }
+        NGW_TRY(operator_apply(ctx, a));
     }
     return NGW_OK;
 }

@@ -52,6 +52,7 @@ enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2, PREC_BF16X3 = 3 };
 struct ngw_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool shared_stream = false;  // stream owned by a batch, not by this context
     // problem
     int n_intp = 0, nband = 0, nval = 0, ncond = 0;
     long ncv = 0;

@@ -78,6 +78,18 @@ int k1_slots(int nv);
 int k1_reduce(hipStream_t s, const double* tpart, long ldt, int nmu, int nvec, int nv, double* t);
 int gemm_lowp(hipStream_t s, int kind, int m, int n, int k, const double* a, long lda, const double* b, long ldb,
               double* c, long ldc);
+// One operator application of a polynomial evaluation:
+//   out = alpha H(x) + bx x + by y + gamma q   (+ out when accumulate)
+struct PolyStage {
+    const double* x = nullptr;
+    double* out = nullptr;
+    const double* y = nullptr;
+    double alpha = 1.0, bx = 0.0, by = 0.0, gamma = 0.0;
+    bool accumulate = false;
+};
+std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const double* q, double* out, double* h0,
+                                 double* h1, double* h2, const double* poly, int npoly, bool accumulate);
+
 // out = p(H) q (+ out when accumulate): Horner in FP64 mode
 // (apply_matvec_BLOCK.F90:250-265), Clenshaw on the Chebyshev form otherwise.
 // h0, h1, h2: workspaces of the same shape as q.
