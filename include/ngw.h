@@ -131,6 +131,18 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
                       int max_cols, double* sigma_c);
 
 /*
+ * The part of ngw_sigma_lanczos contributed by the Green's-function bands
+ * band_lo <= n < band_hi (0-based): the band loop of
+ * SIGMA/calculate_sigma_lanczos_BLOCK.F90z:201-416 restricted to one
+ * contiguous range.  Sigma_c is a plain sum over bands, so ranges computed
+ * on different devices add up to the full result; the unit of work a
+ * multi-device driver would hand to each GPU.
+ */
+int ngw_sigma_lanczos_bands(ngw_context* ctx, int nqp, const int* qp_band, int band_lo, int band_hi, int nen,
+                            double deltae, double ecuts, int niter, const double* polynomial, int npoly,
+                            int tamm_d, int max_cols, double* sigma_c);
+
+/*
  * QP energies by Newton-Raphson on the cubic spline of
  * f(E) = E_dft + Sigma_x + Sigma_c(E) - V_xc - E (eV), replaces the diagonal
  * block of printsigma (SIGMA/printsigma.F90:97-128) with newtonr
@@ -196,10 +208,11 @@ int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, l
  * host, shared/apply_matvec_BLOCK.F90:52-206).  Kernel classes:
  * 0 K1 (C x GEMM), 1 K2 (M t GEMM), 2 K3 (C^T w GEMM + epilogue),
  * 3 Lanczos alpha/beta, 4 continued fraction, 5 P-vector build,
- * 6 Sigma_c assembly.  ngw_profile_read synchronizes and returns the summed
- * milliseconds and launch count of one class since the last reset.
+ * 6 Sigma_c assembly, 7 low-precision operand conversion (x' and W to
+ * FP32/BF16, slot reduction).  ngw_profile_read synchronizes and returns the
+ * summed milliseconds and launch count of one class since the last reset.
  */
-#define NGW_KC_COUNT 7
+#define NGW_KC_COUNT 8
 int ngw_profile_enable(ngw_context* ctx, int on);
 int ngw_profile_reset(ngw_context* ctx);
 int ngw_profile_read(ngw_context* ctx, int kernel_class, double* total_ms, long* launches);

@@ -100,22 +100,29 @@ class GwBatch:
 
 
 def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecuts, hbm_budget_bytes,
-                 sigx=None, use_graph=True, max_active=None, log=None):
+                 sigx=None, use_graph=True, max_active=None, log=None, batch=None, sids=None, polys=None):
     """Continuous-batching request loop.
 
     molecules: list of objects with psi_intp, mmtrx, e1, nv (resident systems)
     requests : list of dicts {"t": arrival time (s), "mol": index, "qp": [bands]}
+    batch, sids, polys: an existing GwBatch with the molecules already added
+               (system ids and sqrt polynomials per molecule), so that operator
+               upload and preparation stay outside a timed serving window
     Returns per-request records with arrival, admit, finish times, Sigma_c and
-    QP energies (Newton on the device).
+    QP energies (Newton on the device), and loop statistics.
     """
-    batch = GwBatch(ctx)
-    sids = [batch.add_system(m.psi_intp, m.mmtrx, m.e1, m.nv) for m in molecules]
-    polys = [lanczos_polynomial(m.e1, m.nv, 12) for m in molecules]
+    own = batch is None
+    if own:
+        batch = GwBatch(ctx)
+        sids = [batch.add_system(m.psi_intp, m.mmtrx, m.e1, m.nv) for m in molecules]
+    if polys is None:
+        polys = [lanczos_polynomial(m.e1, m.nv, 12) for m in molecules]
     pending = sorted(range(len(requests)), key=lambda i: requests[i]["t"])
     queue = []           # arrived, waiting for admission (FIFO by arrival)
     running = {}         # rid -> request index
     used = 0
     recs = [None] * len(requests)
+    s0 = batch.stats()["steps_run"]
     t0 = time.perf_counter()
     pi = 0
     steps = 0
@@ -153,7 +160,7 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
             sx = np.zeros(len(qp)) if sigx is None else sigx[requests[i]["mol"]](qp) * RYD
             res = ctx.qp_newton(sig * RYD, deltae * RYD, e, e, sx, np.zeros(len(qp)))
             recs[i].update({"finish": time.perf_counter() - t0, "sigma": sig, "eqp": res["eqp"],
-                            "ierr": res["ierr"]})
+                            "zfac": res["zfac"], "ierr": res["ierr"]})
             batch.release(rid)
             del running[rid]
             used -= need
@@ -161,5 +168,17 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
                 log(f"[serve] request {i} done: latency {recs[i]['finish'] - recs[i]['arrival']:.3f}s")
     stats = batch.stats()
     stats["batch_steps"] = steps
-    batch.close()
+    stats["lanczos_steps"] = stats["steps_run"] - s0
+    stats["wall_s"] = time.perf_counter() - t0
+    if own:
+        batch.close()
     return recs, stats
+
+
+def latency_stats(recs):
+    """p50 / p99 / max time-to-QP-energies (s) of finished requests."""
+    lat = np.array([r["finish"] - r["arrival"] for r in recs if r is not None and "finish" in r])
+    if lat.size == 0:
+        return {"p50_s": None, "p99_s": None, "max_s": None, "n": 0}
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "max_s": float(lat.max()), "n": int(lat.size)}

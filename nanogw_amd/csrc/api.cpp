@@ -61,11 +61,15 @@ ProfScope::ProfScope(ngw_context* ctx, int k) : c(ctx), kind(k)
     if (a) (void)hipEventRecord(a, c->stream);
 }
 
+static void prof_drain(ngw_context* c);
+
 ProfScope::~ProfScope()
 {
     if (!c || !c->prof_on || !a || !b) return;
     (void)hipEventRecord(b, c->stream);
     c->prof_recs.push_back({kind, a, b});
+    // bound the number of live events on long runs (one sync per 4096 launches)
+    if (c->prof_recs.size() >= 4096) prof_drain(c);
 }
 
 static void prof_drain(ngw_context* c)
@@ -87,7 +91,7 @@ static void prof_drain(ngw_context* c)
 int operator_apply(ngw_context* ctx, const OperatorArgs& a)
 {
     if (ctx->precision == PREC_F64) return operator_f64(ctx, a);
-    return operator_lowp(ctx, a);
+    return operator_lp2(ctx, a);
 }
 
 static int check_ctx(ngw_context* ctx)
@@ -223,7 +227,8 @@ int ngw_destroy(ngw_context* ctx)
     DevBuf* bufs[] = {&ctx->psi, &ctx->mmat, &ctx->e1, &ctx->sqrtr, &ctx->diag_full, &ctx->diag_tda,
                       &ctx->psi32, &ctx->m32, &ctx->psibf, &ctx->mbf, &ctx->psi_scale, &ctx->m_scale,
                       &ctx->tpart, &ctx->ubuf, &ctx->w0, &ctx->w1, &ctx->w2, &ctx->w3, &ctx->w4,
-                      &ctx->ab, &ctx->zz, &ctx->spec, &ctx->misc, &ctx->misc2, &ctx->staging};
+                      &ctx->ab, &ctx->zz, &ctx->spec, &ctx->misc, &ctx->misc2, &ctx->staging,
+                      &ctx->pct32, &ctx->pc32, &ctx->pctb, &ctx->pcb, &ctx->m32t, &ctx->psiv32, &ctx->lpwork};
     for (DevBuf* b : bufs) b->release();
     for (auto& r : ctx->prof_recs) {
         (void)hipEventDestroy(r.a);
@@ -265,6 +270,7 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
     ctx->ld_psi = n_intp;
     ctx->ld_m = n_intp;
     ctx->lowp_ready = false;
+    ctx->lp2_ready = false;
     // compact copies (ld = n_intp)
     std::vector<double> tmp((size_t)n_intp * std::max(nband, n_intp));
     for (int j = 0; j < nband; ++j)
@@ -415,12 +421,23 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
                       int max_cols, double* sigma_c)
 {
     NGW_TRY(check_ctx(ctx));
+    return ngw_sigma_lanczos_bands(ctx, nqp, qp_band, 0, ctx->nband, nen, deltae, ecuts, niter, polynomial, npoly,
+                                   tamm_d, max_cols, sigma_c);
+}
+
+int ngw_sigma_lanczos_bands(ngw_context* ctx, int nqp, const int* qp_band, int band_lo, int band_hi, int nen,
+                            double deltae, double ecuts, int niter, const double* polynomial, int npoly, int tamm_d,
+                            int max_cols, double* sigma_c)
+{
+    NGW_TRY(check_ctx(ctx));
     if (nqp <= 0 || !qp_band || !sigma_c) return fail(NGW_ERR_ARG, "need nqp > 0 and arrays");
     if (nen < 2 || niter < 1 || npoly < 1 || !polynomial) return fail(NGW_ERR_ARG, "bad nen, niter or npoly");
+    if (band_lo < 0 || band_hi > ctx->nband || band_lo >= band_hi)
+        return fail(NGW_ERR_ARG, "need 0 <= band_lo < band_hi <= nband");
     for (int i = 0; i < nqp; ++i)
         if (qp_band[i] < 0 || qp_band[i] >= ctx->nband) return fail(NGW_ERR_ARG, "qp_band out of range");
     hipStream_t s = ctx->stream;
-    const int nb = ctx->nband;
+    const int nb = band_hi - band_lo;
     const long ncv = ctx->ncv;
     const long total = (long)nqp * nb;
     const int nz = nen + 1;
@@ -439,7 +456,7 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
     for (int iq = 0; iq < nqp; ++iq)
         for (int n = 0; n < nb; ++n) {
             mband[(long)iq * nb + n] = qp_band[iq];
-            nband[(long)iq * nb + n] = n;
+            nband[(long)iq * nb + n] = band_lo + n;
         }
     const size_t vb = sizeof(double) * (size_t)ncv * cols;
     NGW_TRY(ctx->w0.ensure(vb));
@@ -479,7 +496,7 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
     }
     {
         ProfScope ps(ctx, KC_SIGMA);
-        NGW_TRY(sigma_assemble(s, spec, zzd, pn2, nen, nqp, nb, ctx->nval, tamm_d ? 1 : 0,
+        NGW_TRY(sigma_assemble(s, spec, zzd, pn2, nen, nqp, nb, band_lo, ctx->nval, tamm_d ? 1 : 0,
                                ctx->misc2.as<double>()));
     }
     NGW_HIP(hipMemcpyAsync(sigma_c, ctx->misc2.ptr, sizeof(double) * 2 * (size_t)nen * nqp,

@@ -30,8 +30,7 @@ namespace ngw {
 // Device descriptor of one request.
 struct GOp {
     const double* psi; long ldp;
-    const float* psi32; const float* sig;
-    const double* m; long ldm; const float* m32;
+    const double* m; long ldm;
     const double* sqrtr; const double* dfull; const double* dtda;
     int nmu, nv, nc, nvec;
     long ld;
@@ -61,25 +60,11 @@ __device__ __forceinline__ int find_req(const int* pre, int nreq, int bid)
 }
 
 constexpr int GK1M = 128, GK1N = 64, GK3M = 64, GK3N = 128, GK2 = 64;
-constexpr int KIND_F64 = -1;
-
-template <int KIND>
-struct K1Smem;
-template <>
-struct K1Smem<KIND_F64> {
-    typedef SmemF64<GK1M, GK1N> type;
-};
-template <int K>
-struct K1Smem {
-    typedef SmemLp<K, GK1M, GK1N> type;
-};
-
-template <int KIND>
 __global__ __launch_bounds__(GNT) void gk1_kernel(const GOp* ops, const GStage* st, int nstage, int s, const int* pre,
                                                   int nreq)
 {
     __shared__ union {
-        typename K1Smem<KIND>::type ab;
+        SmemF64<GK1M, GK1N> ab;
         double red[GK1M][GK1N + 2];
     } sm;
     const int r = find_req(pre, nreq, blockIdx.x);
@@ -97,17 +82,10 @@ __global__ __launch_bounds__(GNT) void gk1_kernel(const GOp* ops, const GStage* 
         if (gm < op.nmu && gn < N) p = val * op.psi[gm + (long)(gn % nv) * op.ldp];
         sm.red[row][col] = p;
     };
-    if constexpr (KIND == KIND_F64) {
-        AccF64<GK1M, GK1N> acc;
-        gemm_f64_tile<GK1M, GK1N>(ColMajorA{op.psi + (long)nv * op.ldp, op.ldp, op.nmu},
-                                  XprimeB{sg.x, sx, op.ld, nv, N}, m0, n0, op.nc, acc, sm.ab);
-        for_acc_f64<GK1M, GK1N>(acc, red_store);
-    } else {
-        AccLp<GK1M, GK1N> acc;
-        gemm_lp_tile<KIND, GK1M, GK1N>(LpColA{op.psi32 + (long)nv * op.nmu, (long)op.nmu, op.nmu},
-                                       LpXprime{sg.x, sx, op.sig + nv, op.ld, nv, N}, m0, n0, op.nc, acc, sm.ab);
-        for_acc_lp<GK1M, GK1N>(acc, [&](int row, int col, float val) { red_store(row, col, (double)val); });
-    }
+    AccF64<GK1M, GK1N> acc;
+    gemm_f64_tile<GK1M, GK1N>(ColMajorA{op.psi + (long)nv * op.ldp, op.ldp, op.nmu}, XprimeB{sg.x, sx, op.ld, nv, N},
+                              m0, n0, op.nc, acc, sm.ab);
+    for_acc_f64<GK1M, GK1N>(acc, red_store);
     __syncthreads();
     const int nend = min(n0 + GK1N, N);
     const int b_lo = n0 / nv, b_hi = (nend - 1) / nv;
@@ -137,7 +115,6 @@ __global__ void gred_kernel(const GOp* ops, const int* pre, int nreq)
     op.t[nu + (long)b * op.nmu] = acc;
 }
 
-template <int KIND>
 __global__ __launch_bounds__(GNT) void gk2_kernel(const GOp* ops, const int* pre, int nreq)
 {
     const int r = find_req(pre, nreq, blockIdx.x);
@@ -149,22 +126,13 @@ __global__ __launch_bounds__(GNT) void gk2_kernel(const GOp* ops, const int* pre
         int gm = m0 + row, gn = n0 + col;
         if (gm < op.nmu && gn < op.nvec) op.u[gm + (long)gn * op.nmu] = val;
     };
-    if constexpr (KIND == KIND_F64) {
-        __shared__ SmemF64<GK2, GK2> sm;
-        AccF64<GK2, GK2> acc;
-        gemm_f64_tile<GK2, GK2>(ColMajorA{op.m, op.ldm, op.nmu}, ColMajorB{op.t, (long)op.nmu, op.nvec}, m0, n0,
-                                op.nmu, acc, sm);
-        for_acc_f64<GK2, GK2>(acc, store);
-    } else {
-        __shared__ SmemLp<LP_F32, GK2, GK2> sm;
-        AccLp<GK2, GK2> acc;
-        gemm_lp_tile<LP_F32, GK2, GK2>(LpColA{op.m32, (long)op.nmu, op.nmu}, LpColBd{op.t, (long)op.nmu, op.nvec}, m0,
-                                       n0, op.nmu, acc, sm);
-        for_acc_lp<GK2, GK2>(acc, [&](int row, int col, float val) { store(row, col, (double)val); });
-    }
+    __shared__ SmemF64<GK2, GK2> sm;
+    AccF64<GK2, GK2> acc;
+    gemm_f64_tile<GK2, GK2>(ColMajorA{op.m, op.ldm, op.nmu}, ColMajorB{op.t, (long)op.nmu, op.nvec}, m0, n0, op.nmu,
+                            acc, sm);
+    for_acc_f64<GK2, GK2>(acc, store);
 }
 
-template <int KIND>
 __global__ __launch_bounds__(GNT) void gk3_kernel(const GOp* ops, const GStage* st, int nstage, int s, const int* pre,
                                                   int nreq)
 {
@@ -188,32 +156,17 @@ __global__ __launch_bounds__(GNT) void gk3_kernel(const GOp* ops, const GStage* 
     ep.acc = acc_on ? sg.out : nullptr; ep.ldacc = op.ld;
     ep.out = sg.out; ep.ldo = op.ld;
     const double* psic = op.psi + (long)nv * op.ldp;
-    if constexpr (KIND == KIND_F64) {
-        __shared__ SmemF64<GK3M, GK3N> sm;
-        AccF64<GK3M, GK3N> acc;
-        gemm_f64_tile<GK3M, GK3N>(ColMajorAT{psic, op.ldp, op.nc}, WeightedPsiVB{op.psi, op.ldp, op.u, (long)op.nmu, nv, N},
-                                  m0, n0, op.nmu, acc, sm);
-        for_acc_f64<GK3M, GK3N>(acc, [&](int row, int col, double val) {
-            int c = m0 + row, j = n0 + col;
-            if (c < op.nc && j < N) {
-                int b = j / nv, v = j - b * nv;
-                ep.apply(v + (long)c * nv, b, val);
-            }
-        });
-    } else {
-        __shared__ SmemLp<KIND, GK3M, GK3N> sm;
-        AccLp<GK3M, GK3N> acc;
-        gemm_lp_tile<KIND, GK3M, GK3N>(LpColAT{op.psi32 + (long)nv * op.nmu, (long)op.nmu, op.nc},
-                                       LpWeighted{op.psi, op.ldp, op.u, (long)op.nmu, nv, N}, m0, n0, op.nmu, acc, sm);
-        const float* sigc = op.sig + nv;
-        for_acc_lp<GK3M, GK3N>(acc, [&](int row, int col, float val) {
-            int c = m0 + row, j = n0 + col;
-            if (c < op.nc && j < N) {
-                int b = j / nv, v = j - b * nv;
-                ep.apply(v + (long)c * nv, b, (double)val * (double)sigc[c]);
-            }
-        });
-    }
+    __shared__ SmemF64<GK3M, GK3N> sm;
+    AccF64<GK3M, GK3N> acc;
+    gemm_f64_tile<GK3M, GK3N>(ColMajorAT{psic, op.ldp, op.nc}, WeightedPsiVB{op.psi, op.ldp, op.u, (long)op.nmu, nv, N},
+                              m0, n0, op.nmu, acc, sm);
+    for_acc_f64<GK3M, GK3N>(acc, [&](int row, int col, double val) {
+        int c = m0 + row, j = n0 + col;
+        if (c < op.nc && j < N) {
+            int b = j / nv, v = j - b * nv;
+            ep.apply(v + (long)c * nv, b, val);
+        }
+    });
 }
 
 constexpr int GRT = 256;
@@ -229,16 +182,22 @@ __device__ __forceinline__ double gblock_sum(double v, double* sh)
     return ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+// Lanczos state of one request, shared by the FP64 and low-precision modes.
+struct GLz {
+    double* q; double* w; double* a; double* b;
+    long ld; long n; int niter; int* kstep;
+};
+
 // Lanczos alpha/beta of step *kstep for every column of every request
 // (lanczos_spectra_BLOCK.F90:63-107, rotation of :81-88 applied here).
-__global__ __launch_bounds__(GRT) void gab_kernel(const GOp* ops, const int* pre, int nreq)
+__global__ __launch_bounds__(GRT) void gab_kernel(const GLz* lz, const int* pre, int nreq)
 {
     __shared__ double sh[4];
     const int r = find_req(pre, nreq, blockIdx.x);
-    const GOp op = ops[r];
+    const GLz op = lz[r];
     const long col = blockIdx.x - pre[r];
     const int k = *op.kstep;
-    const long n = (long)op.nv * op.nc;
+    const long n = op.n;
     double* qc = op.q + col * op.ld;
     double* wc = op.w + col * op.ld;
     double s = 0.0;
@@ -264,10 +223,10 @@ __global__ __launch_bounds__(GRT) void gab_kernel(const GOp* ops, const int* pre
     }
 }
 
-__global__ void gadvance_kernel(const GOp* ops, int nreq)
+__global__ void gadvance_kernel(const GLz* lz, int nreq)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nreq) *ops[r].kstep += 1;
+    if (r < nreq) *lz[r].kstep += 1;
 }
 
 __global__ void set_int_kernel(int* p, int v)
@@ -291,7 +250,7 @@ struct ngw_batch {
         double deltae = 0, ecuts = 0;
         std::vector<double> poly;
         int B = 0;
-        DevBuf vec, work, ab, zz, spec, misc, kstep, sig;
+        DevBuf vec, work, lpwork, ab, zz, spec, misc, kstep, sig;
         int k_host = 1;  // next Lanczos step to run
         bool done = false;
         std::vector<double> sigma;
@@ -301,7 +260,8 @@ struct ngw_batch {
     std::map<int, Request> reqs;
     int next_sys = 1, next_req = 1;
     // grouped launch tables
-    DevBuf gops, gstages, gpre;
+    DevBuf gops, gstages, gpre, glz;
+    int lp_tot[6] = {0, 0, 0, 0, 0, 0};
     std::vector<int> active;  // request ids in the current group
     int nstage = 0;
     int t_k1 = 0, t_red = 0, t_k2 = 0, t_k3 = 0, t_ab = 0;
@@ -326,12 +286,16 @@ size_t request_bytes(const ngw_context* op, int nqp, int niter, int nen)
     const size_t B = (size_t)nqp * op->nband;
     const size_t ncv = op->ncv;
     const size_t S = k1_slots(op->nval);
+    size_t lp = 0;
+    if (op->precision != PREC_F64)
+        lp = lp2_work_bytes(op->precision, op->n_intp, op->nval, op->ncond, (int)B);
     return 8 * (5 * ncv * B + (S + 2) * B * op->n_intp + 2 * (size_t)niter * B + 4 * (size_t)(nen + 1) * B + 4 * B +
-                2 * (size_t)nen * nqp) + 64;
+                2 * (size_t)nen * nqp) + lp + 64;
 }
 
-template <int KIND>
-int launch_step(ngw_batch* b, hipStream_t s)
+// One Lanczos step of every active request.  prof: the context whose kernel
+// timers bracket each kernel class (not used under graph capture).
+int launch_step_f64(ngw_batch* b, hipStream_t s, ngw_context* prof)
 {
     const GOp* ops = b->gops.as<GOp>();
     const GStage* st = b->gstages.as<GStage>();
@@ -341,29 +305,52 @@ int launch_step(ngw_batch* b, hipStream_t s)
     const int* p_red = pre + (nreq + 1);
     const int* p_k2 = pre + 2 * (nreq + 1);
     const int* p_k3 = pre + 3 * (nreq + 1);
-    const int* p_ab = pre + 4 * (nreq + 1);
     for (int stg = 0; stg < b->nstage; ++stg) {
-        hipLaunchKernelGGL(gk1_kernel<KIND>, dim3(b->t_k1), dim3(GNT), 0, s, ops, st, b->nstage, stg, p_k1, nreq);
-        hipLaunchKernelGGL(gred_kernel, dim3(b->t_red), dim3(256), 0, s, ops, p_red, nreq);
-        hipLaunchKernelGGL(gk2_kernel<KIND == KIND_F64 ? KIND_F64 : LP_F32>, dim3(b->t_k2), dim3(GNT), 0, s, ops, p_k2,
-                           nreq);
-        hipLaunchKernelGGL(gk3_kernel<KIND>, dim3(b->t_k3), dim3(GNT), 0, s, ops, st, b->nstage, stg, p_k3, nreq);
+        {
+            ProfScope ps(prof, KC_K1);
+            hipLaunchKernelGGL(gk1_kernel, dim3(b->t_k1), dim3(GNT), 0, s, ops, st, b->nstage, stg, p_k1, nreq);
+        }
+        {
+            ProfScope ps(prof, KC_CONVERT);
+            hipLaunchKernelGGL(gred_kernel, dim3(b->t_red), dim3(256), 0, s, ops, p_red, nreq);
+        }
+        {
+            ProfScope ps(prof, KC_K2);
+            hipLaunchKernelGGL(gk2_kernel, dim3(b->t_k2), dim3(GNT), 0, s, ops, p_k2, nreq);
+        }
+        {
+            ProfScope ps(prof, KC_K3);
+            hipLaunchKernelGGL(gk3_kernel, dim3(b->t_k3), dim3(GNT), 0, s, ops, st, b->nstage, stg, p_k3, nreq);
+        }
     }
-    hipLaunchKernelGGL(gab_kernel, dim3(b->t_ab), dim3(GRT), 0, s, ops, p_ab, nreq);
-    hipLaunchKernelGGL(gadvance_kernel, dim3((nreq + 63) / 64), dim3(64), 0, s, ops, nreq);
     NGW_HIP(hipGetLastError());
     return NGW_OK;
 }
 
-int launch_any(ngw_batch* b, hipStream_t s)
+int launch_any(ngw_batch* b, hipStream_t s, bool capturing)
 {
-    switch (b->ctx->precision) {
-        case PREC_F64: return launch_step<KIND_F64>(b, s);
-        case PREC_F32: return launch_step<LP_F32>(b, s);
-        case PREC_BF16: return launch_step<LP_BF16>(b, s);
-        case PREC_BF16X3: return launch_step<LP_BF16X3>(b, s);
+    const int nreq = (int)b->active.size();
+    const int* pre = b->gpre.as<int>();
+    ngw_context* prof = capturing ? nullptr : b->ctx;
+    if (b->ctx->precision == PREC_F64) {
+        NGW_TRY(launch_step_f64(b, s, prof));
+    } else {
+        const int* p[6];
+        for (int k = 0; k < 6; ++k) p[k] = pre + k * (nreq + 1);
+        LpOp dummy{};
+        LpStage dst{};
+        for (int stg = 0; stg < b->nstage; ++stg)
+            NGW_TRY(lp2_launch_any(b->ctx->precision, s, b->gops.as<LpOp>(), dummy, b->gstages.as<LpStage>(), dst,
+                                   b->nstage, stg, p, b->lp_tot, nreq, prof));
     }
-    return fail(NGW_ERR_STATE, "unknown precision");
+    const int* p_ab = pre + 6 * (nreq + 1);
+    {
+        ProfScope ps(prof, KC_LANCZOS);
+        hipLaunchKernelGGL(gab_kernel, dim3(b->t_ab), dim3(GRT), 0, s, b->glz.as<GLz>(), p_ab, nreq);
+        hipLaunchKernelGGL(gadvance_kernel, dim3((nreq + 63) / 64), dim3(64), 0, s, b->glz.as<GLz>(), nreq);
+    }
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
 }
 
 // Rebuild the descriptor tables for the requests that still have steps to run.
@@ -383,14 +370,19 @@ int rebuild_group(ngw_batch* b)
         if (rq.npoly != npoly) return fail(NGW_ERR_ARG, "all requests in a batch must share npoly");
     }
     b->nstage = npoly;
+    const int prec = b->ctx->precision;
     std::vector<GOp> ops(nreq);
     std::vector<GStage> stages((size_t)nreq * npoly);
-    std::vector<int> pre(5 * (nreq + 1), 0);
+    std::vector<LpOp> lops(nreq);
+    std::vector<LpStage> lstages((size_t)nreq * npoly);
+    std::vector<GLz> lz(nreq);
+    std::vector<int> pre(7 * (nreq + 1), 0);
+    auto P = [&](int k, int i) -> int& { return pre[(size_t)k * (nreq + 1) + i]; };
     for (int i = 0; i < nreq; ++i) {
         auto& rq = b->reqs[b->active[i]];
         ngw_context* op = b->systems[rq.sys].op.get();
         const long ncv = op->ncv;
-        const int B = rq.B, S = k1_slots(op->nval);
+        const int B = rq.B, S = std::max(k1_slots(op->nval), (op->nval - 1) / 64 + 2);
         double* vec = rq.vec.as<double>();
         double* q = vec;
         double* w = vec + (size_t)ncv * B;
@@ -398,51 +390,71 @@ int rebuild_group(ngw_batch* b)
         double* h1 = vec + 3 * (size_t)ncv * B;
         double* h2 = vec + 4 * (size_t)ncv * B;
         double* work = rq.work.as<double>();
-        GOp g;
-        g.psi = op->psi.as<double>(); g.ldp = op->ld_psi;
-        g.psi32 = op->psi32.as<float>(); g.sig = op->psi_scale.as<float>();
-        g.m = op->mmat.as<double>(); g.ldm = op->ld_m; g.m32 = op->m32.as<float>();
-        g.sqrtr = op->sqrtr.as<double>(); g.dfull = op->diag_full.as<double>(); g.dtda = op->diag_tda.as<double>();
-        g.nmu = op->n_intp; g.nv = op->nval; g.nc = op->ncond; g.nvec = B;
-        g.ld = ncv; g.tda = rq.tda;
-        g.tpart = work; g.t = work + (size_t)S * B * op->n_intp; g.u = g.t + (size_t)B * op->n_intp;
-        g.q = q; g.w = w;
-        g.a = rq.ab.as<double>(); g.b = g.a + (size_t)rq.niter * B;
-        g.niter = rq.niter;
-        g.kstep = rq.kstep.as<int>();
-        ops[i] = g;
-        const std::vector<PolyStage> plan =
-            plan_poly(op, b->ctx->precision, q, w, h0, h1, h2, rq.poly.data(), npoly, true);
-        for (int sgi = 0; sgi < npoly; ++sgi) {
-            const PolyStage& p = plan[sgi];
-            GStage gs;
-            gs.x = p.x; gs.out = p.out; gs.y = p.y;
-            gs.alpha = p.alpha; gs.bx = p.bx; gs.by = p.by; gs.gamma = p.gamma;
-            gs.accumulate = p.accumulate ? 1 : 0;
-            stages[(size_t)i * npoly + sgi] = gs;
-        }
+        double* tpart = work;
+        double* t = work + (size_t)S * B * op->n_intp;
+        double* u = t + (size_t)B * op->n_intp;
+        GLz z;
+        z.q = q; z.w = w; z.a = rq.ab.as<double>(); z.b = z.a + (size_t)rq.niter * B;
+        z.ld = ncv; z.n = ncv; z.niter = rq.niter; z.kstep = rq.kstep.as<int>();
+        lz[i] = z;
+        const std::vector<PolyStage> plan = plan_poly(op, prec, q, w, h0, h1, h2, rq.poly.data(), npoly, true);
         const int nmu = op->n_intp, nv = op->nval, nc = op->ncond;
-        const int k1 = ((nmu + GK1M - 1) / GK1M) * ((nv * B + GK1N - 1) / GK1N);
-        const int red = (int)(((long)nmu * B + 255) / 256);
-        const int k2 = ((nmu + GK2 - 1) / GK2) * ((B + GK2 - 1) / GK2);
-        const int k3 = ((nc + GK3M - 1) / GK3M) * ((nv * B + GK3N - 1) / GK3N);
-        pre[0 * (nreq + 1) + i + 1] = pre[0 * (nreq + 1) + i] + k1;
-        pre[1 * (nreq + 1) + i + 1] = pre[1 * (nreq + 1) + i] + red;
-        pre[2 * (nreq + 1) + i + 1] = pre[2 * (nreq + 1) + i] + k2;
-        pre[3 * (nreq + 1) + i + 1] = pre[3 * (nreq + 1) + i] + k3;
-        pre[4 * (nreq + 1) + i + 1] = pre[4 * (nreq + 1) + i] + B;
+        if (prec == PREC_F64) {
+            GOp g;
+            g.psi = op->psi.as<double>(); g.ldp = op->ld_psi;
+            g.m = op->mmat.as<double>(); g.ldm = op->ld_m;
+            g.sqrtr = op->sqrtr.as<double>(); g.dfull = op->diag_full.as<double>(); g.dtda = op->diag_tda.as<double>();
+            g.nmu = nmu; g.nv = nv; g.nc = nc; g.nvec = B;
+            g.ld = ncv; g.tda = rq.tda;
+            g.tpart = tpart; g.t = t; g.u = u;
+            g.q = q; g.w = w; g.a = z.a; g.b = z.b; g.niter = rq.niter; g.kstep = z.kstep;
+            ops[i] = g;
+            for (int sgi = 0; sgi < npoly; ++sgi) {
+                const PolyStage& p = plan[sgi];
+                stages[(size_t)i * npoly + sgi] = GStage{p.x, p.out, p.y, p.alpha, p.bx, p.by, p.gamma,
+                                                         p.accumulate ? 1 : 0};
+            }
+            P(0, i + 1) = P(0, i) + ((nmu + GK1M - 1) / GK1M) * ((nv * B + GK1N - 1) / GK1N);
+            P(1, i + 1) = P(1, i) + (int)(((long)nmu * B + 255) / 256);
+            P(2, i + 1) = P(2, i) + ((nmu + GK2 - 1) / GK2) * ((B + GK2 - 1) / GK2);
+            P(3, i + 1) = P(3, i) + ((nc + GK3M - 1) / GK3M) * ((nv * B + GK3N - 1) / GK3N);
+        } else {
+            NGW_TRY(lp2_prepare(op));
+            NGW_TRY(rq.lpwork.ensure(lp2_work_bytes(prec, nmu, nv, nc, B)));
+            LpOp d;
+            lp2_fill(op, prec, B, ncv, rq.tda, rq.lpwork.as<char>(), tpart, u, d);
+            d.q = q; d.w = w; d.a = z.a; d.b = z.b; d.niter = rq.niter; d.kstep = z.kstep;
+            lops[i] = d;
+            for (int sgi = 0; sgi < npoly; ++sgi) {
+                const PolyStage& p = plan[sgi];
+                lstages[(size_t)i * npoly + sgi] = LpStage{p.x, p.out, p.y, p.alpha, p.bx, p.by, p.gamma,
+                                                           p.accumulate ? 2 : 0};
+            }
+            int tot[6];
+            lp2_grid_counts(prec, d, tot);
+            for (int k = 0; k < 6; ++k) P(k, i + 1) = P(k, i) + tot[k];
+        }
+        P(6, i + 1) = P(6, i) + B;
     }
-    b->t_k1 = pre[1 * (nreq + 1) - 1];
-    b->t_red = pre[2 * (nreq + 1) - 1];
-    b->t_k2 = pre[3 * (nreq + 1) - 1];
-    b->t_k3 = pre[4 * (nreq + 1) - 1];
-    b->t_ab = pre[5 * (nreq + 1) - 1];
     hipStream_t s = b->ctx->stream;
-    NGW_TRY(b->gops.ensure(sizeof(GOp) * ops.size()));
-    NGW_TRY(b->gstages.ensure(sizeof(GStage) * stages.size()));
+    if (prec == PREC_F64) {
+        b->t_k1 = P(0, nreq); b->t_red = P(1, nreq); b->t_k2 = P(2, nreq); b->t_k3 = P(3, nreq);
+        NGW_TRY(b->gops.ensure(sizeof(GOp) * ops.size()));
+        NGW_TRY(b->gstages.ensure(sizeof(GStage) * stages.size()));
+        NGW_HIP(hipMemcpyAsync(b->gops.ptr, ops.data(), sizeof(GOp) * ops.size(), hipMemcpyHostToDevice, s));
+        NGW_HIP(hipMemcpyAsync(b->gstages.ptr, stages.data(), sizeof(GStage) * stages.size(), hipMemcpyHostToDevice, s));
+    } else {
+        for (int k = 0; k < 6; ++k) b->lp_tot[k] = P(k, nreq);
+        NGW_TRY(b->gops.ensure(sizeof(LpOp) * lops.size()));
+        NGW_TRY(b->gstages.ensure(sizeof(LpStage) * lstages.size()));
+        NGW_HIP(hipMemcpyAsync(b->gops.ptr, lops.data(), sizeof(LpOp) * lops.size(), hipMemcpyHostToDevice, s));
+        NGW_HIP(hipMemcpyAsync(b->gstages.ptr, lstages.data(), sizeof(LpStage) * lstages.size(),
+                               hipMemcpyHostToDevice, s));
+    }
+    b->t_ab = P(6, nreq);
+    NGW_TRY(b->glz.ensure(sizeof(GLz) * lz.size()));
     NGW_TRY(b->gpre.ensure(sizeof(int) * pre.size()));
-    NGW_HIP(hipMemcpyAsync(b->gops.ptr, ops.data(), sizeof(GOp) * ops.size(), hipMemcpyHostToDevice, s));
-    NGW_HIP(hipMemcpyAsync(b->gstages.ptr, stages.data(), sizeof(GStage) * stages.size(), hipMemcpyHostToDevice, s));
+    NGW_HIP(hipMemcpyAsync(b->glz.ptr, lz.data(), sizeof(GLz) * lz.size(), hipMemcpyHostToDevice, s));
     NGW_HIP(hipMemcpyAsync(b->gpre.ptr, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, s));
     NGW_HIP(hipStreamSynchronize(s));
     return NGW_OK;
@@ -458,7 +470,7 @@ int finalize(ngw_batch* b, ngw_batch::Request& rq)
     double* bb = a + (size_t)rq.niter * B;
     NGW_TRY(continued_fraction(s, a, bb, rq.niter, B, rq.zz.as<double>(), nz, rq.spec.as<double>()));
     double* pn2 = rq.misc.as<double>();
-    NGW_TRY(sigma_assemble(s, rq.spec.as<double>(), rq.zz.as<double>(), pn2, rq.nen, nqp, op->nband, op->nval, rq.tda,
+    NGW_TRY(sigma_assemble(s, rq.spec.as<double>(), rq.zz.as<double>(), pn2, rq.nen, nqp, op->nband, 0, op->nval, rq.tda,
                            rq.sig.as<double>()));
     rq.sigma.resize(2 * (size_t)rq.nen * nqp);
     NGW_HIP(hipMemcpyAsync(rq.sigma.data(), rq.sig.ptr, sizeof(double) * rq.sigma.size(), hipMemcpyDeviceToHost, s));
@@ -466,6 +478,7 @@ int finalize(ngw_batch* b, ngw_batch::Request& rq)
     rq.done = true;
     rq.vec.release();
     rq.work.release();
+    rq.lpwork.release();
     return NGW_OK;
 }
 
@@ -502,7 +515,7 @@ int ngw_batch_add_system(ngw_batch* b, int n_intp, int nband, int nval, const do
     op->stream = b->ctx->stream;
     op->shared_stream = true;
     int st = ngw_set_isdf(op, n_intp, nband, nval, psi_intp, ld_psi, mmtrx, ld_m, e1);
-    if (st == NGW_OK) st = lowp_prepare(op);
+    if (st == NGW_OK && b->ctx->precision != PREC_F64) st = lp2_prepare(op);
     if (st != NGW_OK) {
         ngw_destroy(op);
         return st;
@@ -559,7 +572,7 @@ int ngw_batch_submit(ngw_batch* b, int sys_id, int nqp, const int* qp_band, int 
     const long ncv = op->ncv;
     rq.B = B;
     rq.bytes = request_bytes(op, nqp, niter, nen);
-    const int S = k1_slots(op->nval);
+    const int S = std::max(k1_slots(op->nval), (op->nval - 1) / 64 + 2);
     int st = NGW_OK;
     auto fail_req = [&](int code) {
         b->reqs.erase(id);
@@ -612,7 +625,7 @@ int ngw_batch_step(ngw_batch* b, int nsteps, int use_graph, int* n_finished)
         if (use_graph) {
             if (!b->gexec) {
                 NGW_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                int st = launch_any(b, s);
+                int st = launch_any(b, s, true);
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamEndCapture(s, &g);
                 if (st != NGW_OK) {
@@ -626,7 +639,7 @@ int ngw_batch_step(ngw_batch* b, int nsteps, int use_graph, int* n_finished)
             }
             NGW_HIP(hipGraphLaunch(b->gexec, s));
         } else {
-            NGW_TRY(launch_any(b, s));
+            NGW_TRY(launch_any(b, s, false));
         }
         ++b->steps_run;
         for (int id : b->active) {

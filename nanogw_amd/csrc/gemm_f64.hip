@@ -173,6 +173,7 @@ static int launch_k1(ngw_context* ctx, const double* x, long ldx, const double* 
                            mt * nt);
         NGW_HIP(hipGetLastError());
     }
+    ProfScope ps(ctx, KC_CONVERT);
     return k1_reduce(s, tpart, (long)nmu, nmu, B, nv, t);
 }
 

@@ -334,9 +334,10 @@ int zz_grid(hipStream_t s, const double* d_e1, const int* d_mband, const int* d_
 //   full, empty   : 2|P|^2 / zeta (S(zeta) - S(0))      (:408-409)
 //   TDA           : -S(zeta) eta sqrt(2) |P|^2          (lowcomsym :738-740; the
 //                   reference's 2.0**0.5 is single precision)
-// Columns are laid out as col = iq * nband + n.
+// Columns are laid out as col = iq * nband + n for the bands band0 + n,
+// n < nband (a contiguous band range; the full sum is band0 = 0, nband = N_b).
 __global__ void sigma_assemble_kernel(const double* spec, const double* zz, const double* pn2,
-                                      int nen, int nqp, int nband, int nval, int tda, double* sigma)
+                                      int nen, int nqp, int nband, int band0, int nval, int tda, double* sigma)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nen * nqp) return;
@@ -349,7 +350,7 @@ __global__ void sigma_assemble_kernel(const double* spec, const double* zz, cons
         const long o = 2 * (i + col * nz), o0 = 2 * (nen + col * nz);
         const cplx si = {spec[o], spec[o + 1]}, s0 = {spec[o0], spec[o0 + 1]};
         const double p2 = pn2[col];
-        const bool occ = n < nval;
+        const bool occ = band0 + n < nval;
         cplx inc;
         if (tda) {
             const double eta = occ ? 1.0 : -1.0;
@@ -368,11 +369,11 @@ __global__ void sigma_assemble_kernel(const double* spec, const double* zz, cons
 }
 
 int sigma_assemble(hipStream_t s, const double* d_spec, const double* d_zz, const double* d_pn2,
-                   int nen, int nqp, int nband, int nval, int tda, double* d_sigma)
+                   int nen, int nqp, int nband, int band0, int nval, int tda, double* d_sigma)
 {
     int total = nen * nqp;
     hipLaunchKernelGGL(sigma_assemble_kernel, dim3((total + 127) / 128), dim3(128), 0, s, d_spec, d_zz,
-                       d_pn2, nen, nqp, nband, nval, tda, d_sigma);
+                       d_pn2, nen, nqp, nband, band0, nval, tda, d_sigma);
     NGW_HIP(hipGetLastError());
     return NGW_OK;
 }

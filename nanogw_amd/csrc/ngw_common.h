@@ -40,7 +40,7 @@ struct DevBuf {
 };
 
 // Kernel classes timed by the optional per-launch HIP-event profiler.
-enum KernelClass { KC_K1 = 0, KC_K2, KC_K3, KC_LANCZOS, KC_CFRAC, KC_PVEC, KC_SIGMA, KC_COUNT };
+enum KernelClass { KC_K1 = 0, KC_K2, KC_K3, KC_LANCZOS, KC_CFRAC, KC_PVEC, KC_SIGMA, KC_CONVERT, KC_COUNT };
 
 // Precision of the three GEMM stages of the operator.
 enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2, PREC_BF16X3 = 3 };
@@ -69,6 +69,10 @@ struct ngw_context {
     // low-precision operator copies (FP32 / BF16 with per-band scales)
     ngw::DevBuf psi32, m32, psibf, mbf, psi_scale, m_scale;
     bool lowp_ready = false;
+    // pre-converted operands of the streamlined low-precision path
+    bool lp2_ready = false;
+    int ldc1 = 0, ldm1 = 0;
+    ngw::DevBuf pct32, pc32, pctb, pcb, m32t, psiv32, lpwork;
     // workspaces
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;
     // host staging for drop-in calls

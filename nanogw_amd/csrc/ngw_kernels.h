@@ -103,6 +103,38 @@ int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int n
 int operator_apply(ngw_context* ctx, const OperatorArgs& a);
 int lowp_prepare(ngw_context* ctx);
 int operator_lowp(ngw_context* ctx, const OperatorArgs& a);
+// Low-precision request descriptor (the operator of one system plus one
+// block of vectors) for the pre-converted-operand kernels.
+struct LpOp {
+    const void* pct_hi; const void* pct_lo;
+    const void* pc_hi; const void* pc_lo;
+    const float* m32t; const float* psiv32;
+    const double* psi; long ldp;
+    const float* sig;
+    const double* sqrtr; const double* dfull; const double* dtda;
+    int nmu, nv, nc, nvec, ldc1, ldm1;
+    long ld; int tda;
+    void* xlp_hi; void* xlp_lo; void* wlp_hi; void* wlp_lo;
+    float* t32; double* tpart; double* u;
+    double* q; double* w; double* a; double* b; int niter; int* kstep;
+};
+
+// Stage of a polynomial evaluation (see PolyStage).  accumulate: 0 no,
+// 1 yes, 2 only once the request's Lanczos step counter is past 1.
+struct LpStage {
+    const double* x; double* out; const double* y;
+    double alpha, bx, by, gamma;
+    int accumulate;
+};
+
+int lp2_prepare(ngw_context* ctx);
+int lp2_launch_any(int kind, hipStream_t st, const LpOp* ops, const LpOp& op1, const LpStage* stg, const LpStage& st1,
+                   int nstage, int s, const int* const* pre, const int* tot, int nreq, ngw_context* prof);
+size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec);
+void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char* work, double* tpart, double* u,
+              LpOp& d);
+void lp2_grid_counts(int kind, const LpOp& d, int* tot);
+int operator_lp2(ngw_context* ctx, const OperatorArgs& a);
 
 // Lanczos pieces (lanczos.hip).
 int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double* d_h1, double* d_h2,
@@ -116,8 +148,8 @@ int continued_fraction(hipStream_t s, const double* d_a, const double* d_b, int 
                        const double* d_zz, int nz, double* d_spec);
 int zz_grid(hipStream_t s, const double* d_e1, const int* d_mband, const int* d_nband, int nval,
             int nvec, int nen, double deltae, double ecuts, double* d_zz);
-int sigma_assemble(hipStream_t s, const double* d_spec, const double* d_zz, const double* d_pn2,
-                   int nen, int nqp, int nband, int nval, int tda, double* d_sigma);
+int sigma_assemble(hipStream_t s, const double* d_spec, const double* d_zz, const double* d_pn2, int nen, int nqp,
+                   int nband, int band0, int nval, int tda, double* d_sigma);
 int qp_newton(hipStream_t s, int nqp, int nen, double deltae_ev, const double* d_in,
               double* d_out, double* d_scratch);
 

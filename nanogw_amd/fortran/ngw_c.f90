@@ -90,6 +90,18 @@ module ngw_c
       complex(c_double_complex), intent(out) :: sigma_c(*)
     end function ngw_sigma_lanczos
 
+    integer(c_int) function ngw_sigma_lanczos_bands(ctx, nqp, qp_band, band_lo, band_hi, nen, deltae, &
+                                                    ecuts, niter, polynomial, npoly, tamm_d, max_cols, &
+                                                    sigma_c) bind(C, name='ngw_sigma_lanczos_bands')
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nqp, band_lo, band_hi, nen, niter, npoly, tamm_d, max_cols
+      integer(c_int), intent(in) :: qp_band(*)
+      real(c_double), value :: deltae, ecuts
+      real(c_double), intent(in) :: polynomial(*)
+      complex(c_double_complex), intent(out) :: sigma_c(*)
+    end function ngw_sigma_lanczos_bands
+
     integer(c_int) function ngw_qp_newton(ctx, nqp, nen, deltae_ev, edft, en0, sigx, vxc, &
                                           sigma_c, eqp, zfac, sigma0, imsig, ierr) &
       bind(C, name='ngw_qp_newton')

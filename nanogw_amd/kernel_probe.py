@@ -1,0 +1,73 @@
+"""Short fixed-shape driver of the operator kernels, for profiler runs.
+
+Builds a random ISDF operator of the given sizes on the GPU (Psi at the ISDF
+points ~ N(0, 1/Nmu), M a random symmetric positive matrix), applies the
+operator to --nvec vectors --reps times in the chosen precision through the
+device-pointer entry point, then runs one plain device-to-device copy of
+--copy-bytes (the reference kernel that converts a hardware byte counter
+into a scale factor).  Kernel timings and counters come from the profiler
+wrapped around this command; the script prints what it launched.
+
+  python -m nanogw_amd.kernel_probe --nmu 5000 --nv 100 --nc 1500 \\
+      --nvec 1600 --precision bf16 --reps 3
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nmu", type=int, default=5000)
+    ap.add_argument("--nv", type=int, default=100)
+    ap.add_argument("--nc", type=int, default=1500)
+    ap.add_argument("--nvec", type=int, default=1600)
+    ap.add_argument("--precision", default="bf16", choices=["f64", "f32", "bf16", "bf16x3"])
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--tda", action="store_true")
+    a = ap.parse_args(argv)
+
+    import numpy as np
+    import torch
+
+    from .lanczos import IsdfContext
+
+    rng = np.random.default_rng(1)
+    nb = a.nv + a.nc
+    psi = rng.normal(size=(a.nmu, nb)) / np.sqrt(a.nmu)
+    g = rng.normal(size=(a.nmu, a.nmu)) / a.nmu
+    mm = g @ g.T + np.eye(a.nmu)
+    e1 = np.concatenate([np.sort(rng.uniform(-1.8, -0.5, a.nv)), np.sort(rng.uniform(-0.3, 4.0, a.nc))])
+    ctx = IsdfContext(0)
+    ctx.set_isdf(np.asfortranarray(psi), np.asfortranarray(mm), e1, a.nv)
+    ctx.set_precision(a.precision)
+    ncv = a.nv * a.nc
+    dev = torch.device("cuda:0")
+    x = torch.randn((a.nvec, ncv), dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, a.nvec, a.tda)  # first call prepares the operands
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(a.reps):
+        ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, a.nvec, a.tda)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t) / a.reps
+    print(f"[probe] operator {a.precision} Nmu={a.nmu} nv={a.nv} nc={a.nc} nvec={a.nvec}: "
+          f"{a.reps + 1} applications, {dt * 1e3:.3f} ms each after the first", flush=True)
+    n = a.copy_bytes // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[probe] device copy of {n * 8} bytes (read {n * 8}, write {n * 8})", flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

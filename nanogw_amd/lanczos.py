@@ -140,16 +140,24 @@ class IsdfContext:
         return spec, a, bb
 
     def calculate_sigma_lanczos(self, qp, nen, deltae, ecuts, niter, polynomial, tamm_d=False,
-                                max_cols=0):
+                                max_cols=0, bands=None):
         """Sigma_c(z) (nen, nqp) complex in Ry for 0-based QP bands `qp`
-        (calculate_sigma_lanczos_BLOCK.F90z:201-444)."""
+        (calculate_sigma_lanczos_BLOCK.F90z:201-444).  bands = (lo, hi)
+        restricts the sum to Green's-function bands lo <= n < hi."""
         qp = np.ascontiguousarray(qp, np.int32)
         poly = _f64(polynomial)
         out = np.zeros((nen, len(qp)), np.complex128, order="F")
-        _lib.check(self._lib.ngw_sigma_lanczos(
-            self._h, len(qp), qp.ctypes.data_as(C.POINTER(C.c_int)), int(nen), float(deltae),
-            float(ecuts), int(niter), _p(poly), len(poly) - 1, int(tamm_d), int(max_cols),
-            out.ctypes.data_as(C.POINTER(C.c_double))), "ngw_sigma_lanczos")
+        qpp = qp.ctypes.data_as(C.POINTER(C.c_int))
+        outp = out.ctypes.data_as(C.POINTER(C.c_double))
+        if bands is None:
+            _lib.check(self._lib.ngw_sigma_lanczos(
+                self._h, len(qp), qpp, int(nen), float(deltae), float(ecuts), int(niter), _p(poly),
+                len(poly) - 1, int(tamm_d), int(max_cols), outp), "ngw_sigma_lanczos")
+        else:
+            lo, hi = bands
+            _lib.check(self._lib.ngw_sigma_lanczos_bands(
+                self._h, len(qp), qpp, int(lo), int(hi), int(nen), float(deltae), float(ecuts), int(niter),
+                _p(poly), len(poly) - 1, int(tamm_d), int(max_cols), outp), "ngw_sigma_lanczos_bands")
         return out
 
     def qp_newton(self, sigma_c_ev, deltae_ev, edft_ev, en0_ev, sigx_ev, vxc_ev):
@@ -195,7 +203,7 @@ class IsdfContext:
                    "ngw_gemm_f64_dev")
 
     # ------------------------------------------------------------ profiling
-    KERNEL_CLASSES = ("K1", "K2", "K3", "lanczos_ab", "cfrac", "pvectors", "sigma")
+    KERNEL_CLASSES = ("K1", "K2", "K3", "lanczos_ab", "cfrac", "pvectors", "sigma", "convert")
 
     def profile_enable(self, on=True):
         _lib.check(self._lib.ngw_profile_enable(self._h, int(bool(on))), "ngw_profile_enable")

@@ -98,8 +98,8 @@ def fft_box(gidx: np.ndarray, h: float):
     return rmax, (n, n, n), 2.0 * rmax
 
 
-def _hermite(n: int, x: np.ndarray) -> np.ndarray:
-    h0 = np.ones_like(x)
+def _hermite(n: int, x):
+    h0 = x * 0.0 + 1.0
     if n == 0:
         return h0
     h1 = 2.0 * x
@@ -111,25 +111,55 @@ def _hermite(n: int, x: np.ndarray) -> np.ndarray:
 _DEGS = [(a, b, c) for a in range(5) for b in range(5) for c in range(5) if a + b + c <= 4]
 
 
-def orbitals(gidx, h, nb, rng, radius, sigma=2.5, nterms=6, chunk=256):
-    """(ngrid, nb) grid-normalised orthonormal orbitals (QR with weight h^3)."""
+def _orbital_draws(nb, rng, radius, nterms):
+    """Seeded per-orbital parameters: centre offset, Hermite terms, coefficients."""
+    out = []
+    for _ in range(nb):
+        d = rng.normal(size=3)
+        d *= rng.uniform(0.0, 0.5 * radius) / max(np.linalg.norm(d), 1e-12)
+        pick = rng.choice(len(_DEGS), size=nterms, replace=False)
+        coef = rng.normal(size=nterms)
+        out.append((d, [_DEGS[i] for i in pick], coef))
+    return out
+
+
+def orbitals(gidx, h, nb, rng, radius, sigma=2.5, nterms=6, device=None):
+    """(ngrid, nb) grid-normalised orthonormal orbitals (QR with weight h^3).
+
+    device: a torch device on which the orbitals are evaluated and
+    QR-factorised in FP64 (large systems); numpy otherwise.  The random
+    draws are the same either way.
+    """
+    draws = _orbital_draws(nb, rng, radius, nterms)
+    if device is not None:
+        import torch
+
+        r = torch.as_tensor(gidx.astype(np.float64) * h, device=device)
+        phi = torch.empty((nb, r.shape[0]), dtype=torch.float64, device=device)
+        for j, (d, degs, coef) in enumerate(draws):
+            u = (r - torch.as_tensor(d, device=device)) / sigma
+            env = torch.exp(-0.5 * (u * u).sum(dim=1))
+            poly = torch.zeros_like(env)
+            for (a, b, c), cf in zip(degs, coef):
+                poly += float(cf) * _hermite(a, u[:, 0]) * _hermite(b, u[:, 1]) * _hermite(c, u[:, 2])
+            phi[j] = env * poly
+        q, rr = torch.linalg.qr(phi.T * math.sqrt(h ** 3))
+        del phi
+        q *= torch.sign(torch.diagonal(rr))[None, :]
+        out = np.asfortranarray(q.cpu().numpy())
+        del q, rr
+        torch.cuda.empty_cache()
+        return out
     r = gidx.astype(np.float64) * h
     ng = r.shape[0]
     phi = np.empty((ng, nb), order="F")
-    for j0 in range(0, nb, chunk):
-        for j in range(j0, min(nb, j0 + chunk)):
-            d = rng.normal(size=3)
-            d *= rng.uniform(0.0, 0.5 * radius) / max(np.linalg.norm(d), 1e-12)
-            u = (r - d) / sigma
-            env = np.exp(-0.5 * (u * u).sum(axis=1))
-            poly = np.zeros(ng)
-            pick = rng.choice(len(_DEGS), size=nterms, replace=False)
-            coef = rng.normal(size=nterms)
-            for t, idx in enumerate(pick):
-                a, b, c = _DEGS[idx]
-                poly += coef[t] * _hermite(a, u[:, 0]) * _hermite(b, u[:, 1]) * _hermite(c, u[:, 2])
-            phi[:, j] = env * poly
-    # QR with weight h^3: psi = phi sqrt(h^3) orthonormalised -> sum psi^2 = 1
+    for j, (d, degs, coef) in enumerate(draws):
+        u = (r - d) / sigma
+        env = np.exp(-0.5 * (u * u).sum(axis=1))
+        poly = np.zeros(ng)
+        for (a, b, c), cf in zip(degs, coef):
+            poly += cf * _hermite(a, u[:, 0]) * _hermite(b, u[:, 1]) * _hermite(c, u[:, 2])
+        phi[:, j] = env * poly
     q, rr = np.linalg.qr(phi * math.sqrt(h ** 3))
     q *= np.sign(np.diag(rr))[None, :]
     return np.asfortranarray(q)
@@ -210,12 +240,19 @@ def isdf_zeta(psi, intp, nv, ridge=0.0, device=None):
             A = A + ridge * torch.trace(A) / A.shape[0] * torch.eye(A.shape[0], dtype=A.dtype, device=device)
         B = (pv @ pi_v.T) * (t @ pi_n.T)
         del t, pv
-        lu, piv = torch.linalg.lu_factor(A)
+        # A is symmetric: zeta = B A^{-1}.  The n_intp x n_intp inverse is formed
+        # on the host (LU with partial pivoting, scipy); the large product is a
+        # plain GPU GEMM (device triangular solves need a library workspace
+        # that torch caps below what n_intp = 5000 asks for).
+        import scipy.linalg as sla
+
+        ainv = sla.inv(A.cpu().numpy(), check_finite=False)
+        ainv = torch.as_tensor(0.5 * (ainv + ainv.T), device=device)
         out = np.empty((B.shape[0], A.shape[0]), order="F")
-        for r0 in range(0, B.shape[0], 8192):
-            zt = torch.linalg.lu_solve(lu, piv, B[r0:r0 + 8192].T.contiguous())
-            out[r0:r0 + zt.shape[1], :] = zt.T.cpu().numpy()
-        del A, B, lu, piv
+        for r0 in range(0, B.shape[0], 32768):
+            zt = B[r0:r0 + 32768] @ ainv
+            out[r0:r0 + zt.shape[0], :] = zt.cpu().numpy()
+        del A, B, ainv
         torch.cuda.empty_cache()
         return out
     pv = psi[:, :nv]
@@ -268,7 +305,7 @@ def make_molecule(radius, h, nv, nc, n_intp, seed, with_kernel=True, keep_psi=Fa
     gidx = sphere_grid(radius, h)
     rmax, nfft, trunc = fft_box(gidx, h)
     nb = nv + nc
-    psi = orbitals(gidx, h, nb, rng, radius)
+    psi = orbitals(gidx, h, nb, rng, radius, device=device)
     e1 = energies(nv, nc, rng)
     rho = 2.0 * (psi[:, :nv] ** 2).sum(axis=1)
     intp = cvt_points(gidx, h, rho, n_intp, rng, max_iter=cvt_iter)
@@ -295,6 +332,50 @@ CONFIGS = {
     "c2": (10.0, 0.35, 40, 400, 2000, 300, 8),
     "c3": (13.0, 0.33, 100, 1500, 5000, 300, 8),
 }
+
+
+def c4_specs(n_mol=64, seed=None):
+    """Molecule sizes of the batched workload: R ~ U[6, 11] bohr, h = 0.4,
+    nv ~ U{8..60}, nc = 10 nv, Nmu = 12 nv, niter = 250, 16 QP states each.
+    Returns a list of dicts with the make_molecule arguments."""
+    rng = np.random.default_rng(config_seed(4, 0) if seed is None else seed)
+    out = []
+    for i in range(n_mol):
+        nv = int(rng.integers(8, 61))
+        out.append({"radius": float(rng.uniform(6.0, 11.0)), "h": 0.4, "nv": nv, "nc": 10 * nv,
+                    "n_intp": 12 * nv, "seed": config_seed(4, i), "nqp": 16, "niter": 250})
+    return out
+
+
+def c5_stream(duration_s, rate=20.0, seed=None, large_frac=0.1, n_small=64):
+    """Poisson request stream: exponential gaps at `rate` req/s over
+    duration_s; each request picks one of the n_small c4-distributed
+    molecules, or with probability large_frac the R = 13, nv = 100 molecule
+    (index n_small), and asks for U{4..16} QP states around the gap.
+    Returns (requests, specs): requests as {"t", "mol", "nqp"}."""
+    rng = np.random.default_rng(config_seed(5, 0) if seed is None else seed)
+    specs = c4_specs(n_small, seed=config_seed(5, 1) if seed is None else seed + 1)
+    R, h, nv, nc, nmu, niter, _ = CONFIGS["c3"]
+    specs.append({"radius": R, "h": h, "nv": nv, "nc": nc, "n_intp": nmu, "seed": config_seed(5, 999),
+                  "nqp": 8, "niter": niter})
+    reqs, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration_s:
+            break
+        big = rng.uniform() < large_frac
+        mol = n_small if big else int(rng.integers(0, n_small))
+        reqs.append({"t": t, "mol": mol, "nqp": int(rng.integers(4, 17))})
+    return reqs, specs
+
+
+def qp_window(nv, nb, nqp):
+    """nqp QP states around the gap (HOMO-(nqp/2-1) .. LUMO+(nqp - nqp/2 - 1)),
+    clipped to the available bands."""
+    lo = max(0, nv - nqp // 2)
+    hi = min(nb, lo + nqp)
+    lo = max(0, hi - nqp)
+    return list(range(lo, hi))
 
 
 def qp_states(nv, nqp):

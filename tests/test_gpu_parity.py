@@ -99,6 +99,24 @@ def test_sigma_lanczos_mini_vs_oracle(ngw_ctx, mini):
     assert rel_err(sig, ref) < 1e-6
 
 
+@pytest.mark.parametrize("tda", [False, True])
+def test_sigma_band_ranges_add_up(ngw_ctx, mini, tda):
+    """Sigma_c over band ranges (occupied part, two empty parts, a range that
+    straddles the gap) sums to the full band sum: the per-device unit of a
+    band-sharded run."""
+    ngw_ctx.set_isdf(mini.psi_intp, mini.mmtrx, mini.e1, mini.nv)
+    poly = np.array([1.0, 0.0]) if tda else _poly(mini)
+    qp = [mini.nv - 1, mini.nv]
+    args = (qp, 31, 4.0, 0.02, 20, poly)
+    full = ngw_ctx.calculate_sigma_lanczos(*args, tamm_d=tda)
+    nv, nb = mini.nv, mini.nv + mini.nc
+    for cuts in ([0, nv, nv + 5, nb], [0, nv - 1, nv + 2, nb]):
+        parts = [ngw_ctx.calculate_sigma_lanczos(*args, tamm_d=tda, bands=(lo, hi))
+                 for lo, hi in zip(cuts[:-1], cuts[1:])]
+        assert all(np.abs(p).max() > 0 for p in parts)
+        assert rel_err(sum(parts), full) < 1e-12, cuts
+
+
 def test_sigma_lanczos_tda_vs_oracle(ngw_ctx, mini_tda):
     from oracle import oracle as O
 
