@@ -203,6 +203,19 @@ int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, l
                      const double* d_b, long ldb, double* d_c, long ldc);
 
 /*
+ * Tile-shape autotuner of the low-precision operator GEMMs (no NanoGW
+ * counterpart: NanoGW calls vendor BLAS).  For the context's current
+ * precision and operator sizes, applies the operator to nvec vectors with
+ * each of the four {BM, BN, slabs-in-flight} configurations of K1 (C x) and
+ * K3 (C^T w), reps timed applications each, keeps the fastest of each for
+ * later calls on this context (and batches created from it) and returns the
+ * chosen indices.  A no-op in FP64 mode.  ngw_set_tile_config sets the
+ * indices (0..3) directly; index 0 is the default.
+ */
+int ngw_autotune(ngw_context* ctx, int nvec, int reps, int* k1_cfg, int* k3_cfg);
+int ngw_set_tile_config(ngw_context* ctx, int k1_cfg, int k3_cfg);
+
+/*
  * Per-launch kernel timing with HIP events on the context's stream (no
  * NanoGW counterpart; NanoGW's timacc slots 68-78 time the same stages on the
  * host, shared/apply_matvec_BLOCK.F90:52-206).  Kernel classes:

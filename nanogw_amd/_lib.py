@@ -46,6 +46,8 @@ _SIGS = {
     "ngw_profile_enable": ([_vp, C.c_int], C.c_int),
     "ngw_profile_reset": ([_vp], C.c_int),
     "ngw_profile_read": ([_vp, C.c_int, _dp, C.POINTER(C.c_long)], C.c_int),
+    "ngw_autotune": ([_vp, C.c_int, C.c_int, _ip, _ip], C.c_int),
+    "ngw_set_tile_config": ([_vp, C.c_int, C.c_int], C.c_int),
     "ngw_gemm_lowp_dev": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_long, _vp, C.c_long, _vp,
                            C.c_long], C.c_int),
     "ngw_batch_create": ([_vp, C.POINTER(_vp)], C.c_int),

@@ -11,10 +11,14 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libngw.so")
 ARCH = os.environ.get("NGW_OFFLOAD_ARCH", "gfx950")
 
-# Packed FP32 instructions are disabled for every device compile.
+# Packed FP32 instructions are disabled for every device compile.  MFMA
+# accumulators are allowed in VGPRs (gfx90a and later): with accumulators
+# pinned to AGPRs the compiler rotates them through v_accvgpr_mov copies in
+# the unrolled GEMM loops.
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 

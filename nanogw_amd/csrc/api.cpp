@@ -225,7 +225,7 @@ int ngw_destroy(ngw_context* ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->psi, &ctx->mmat, &ctx->e1, &ctx->sqrtr, &ctx->diag_full, &ctx->diag_tda,
-                      &ctx->psi32, &ctx->m32, &ctx->psibf, &ctx->mbf, &ctx->psi_scale, &ctx->m_scale,
+                      &ctx->psi_scale,
                       &ctx->tpart, &ctx->ubuf, &ctx->w0, &ctx->w1, &ctx->w2, &ctx->w3, &ctx->w4,
                       &ctx->ab, &ctx->zz, &ctx->spec, &ctx->misc, &ctx->misc2, &ctx->staging,
                       &ctx->pct32, &ctx->pc32, &ctx->pctb, &ctx->pcb, &ctx->m32t, &ctx->psiv32, &ctx->lpwork};
@@ -324,6 +324,68 @@ int ngw_matvec_isdf_block_dev(ngw_context* ctx, const double* d_vec, double* d_h
     a.x = d_vec; a.ldx = ld; a.nvec = blksz; a.tda = tamm_d ? 1 : 0;
     a.out = d_hvec; a.ldo = ld;
     return operator_apply(ctx, a);
+}
+
+int ngw_autotune(ngw_context* ctx, int nvec, int reps, int* k1_cfg, int* k3_cfg)
+{
+    NGW_TRY(check_ctx(ctx));
+    if (nvec <= 0 || reps <= 0) return fail(NGW_ERR_ARG, "need nvec > 0 and reps > 0");
+    const int prec = ctx->precision;
+    if (prec == PREC_F64) {
+        if (k1_cfg) *k1_cfg = 0;
+        if (k3_cfg) *k3_cfg = 0;
+        return NGW_OK;
+    }
+    NGW_TRY(lp2_prepare(ctx));
+    hipStream_t s = ctx->stream;
+    const size_t bytes = sizeof(double) * (size_t)ctx->ncv * nvec;
+    DevBuf x, y;
+    NGW_TRY(x.ensure(bytes));
+    NGW_TRY(y.ensure(bytes));
+    // any finite values time the same; 0x3f bytes make every double 4.8e-4
+    NGW_HIP(hipMemsetAsync(x.ptr, 0x3f, bytes, s));
+    prof_drain(ctx);
+    const bool was_on = ctx->prof_on;
+    double keep_ms[KC_COUNT];
+    long keep_cnt[KC_COUNT];
+    std::copy(ctx->prof_ms, ctx->prof_ms + KC_COUNT, keep_ms);
+    std::copy(ctx->prof_cnt, ctx->prof_cnt + KC_COUNT, keep_cnt);
+    ctx->prof_on = true;
+    OperatorArgs a;
+    a.x = x.as<double>(); a.ldx = ctx->ncv; a.nvec = nvec; a.out = y.as<double>(); a.ldo = ctx->ncv;
+    double best1 = 1e300, best3 = 1e300;
+    int b1 = 0, b3 = 0, st = NGW_OK;
+    for (int c = 0; c < 4 && st == NGW_OK; ++c) {
+        ctx->lp_cfg[prec][0] = c;
+        ctx->lp_cfg[prec][1] = c;
+        st = operator_apply(ctx, a);  // first use of a configuration
+        prof_drain(ctx);
+        std::fill(ctx->prof_ms, ctx->prof_ms + KC_COUNT, 0.0);
+        for (int r = 0; r < reps && st == NGW_OK; ++r) st = operator_apply(ctx, a);
+        prof_drain(ctx);
+        if (ctx->prof_ms[KC_K1] < best1) { best1 = ctx->prof_ms[KC_K1]; b1 = c; }
+        if (ctx->prof_ms[KC_K3] < best3) { best3 = ctx->prof_ms[KC_K3]; b3 = c; }
+    }
+    ctx->lp_cfg[prec][0] = b1;
+    ctx->lp_cfg[prec][1] = b3;
+    std::copy(keep_ms, keep_ms + KC_COUNT, ctx->prof_ms);
+    std::copy(keep_cnt, keep_cnt + KC_COUNT, ctx->prof_cnt);
+    ctx->prof_on = was_on;
+    NGW_HIP(hipStreamSynchronize(s));
+    if (st != NGW_OK) return st;
+    if (k1_cfg) *k1_cfg = b1;
+    if (k3_cfg) *k3_cfg = b3;
+    return NGW_OK;
+}
+
+int ngw_set_tile_config(ngw_context* ctx, int k1_cfg, int k3_cfg)
+{
+    if (!ctx) return fail(NGW_ERR_ARG, "null context");
+    if (k1_cfg < 0 || k1_cfg > 3 || k3_cfg < 0 || k3_cfg > 3) return fail(NGW_ERR_ARG, "configuration index 0..3");
+    if (ctx->precision == PREC_F64) return NGW_OK;
+    ctx->lp_cfg[ctx->precision][0] = k1_cfg;
+    ctx->lp_cfg[ctx->precision][1] = k3_cfg;
+    return NGW_OK;
 }
 
 int ngw_matvec_isdf_block(ngw_context* ctx, const double* vec, double* hvec, int ncv, int blksz,

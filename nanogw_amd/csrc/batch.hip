@@ -60,6 +60,9 @@ __device__ __forceinline__ int find_req(const int* pre, int nreq, int bid)
 }
 
 constexpr int GK1M = 128, GK1N = 64, GK3M = 64, GK3N = 128, GK2 = 64;
+// Workgroup prefix tables of a step: one per kernel of the operator (FP64
+// uses the first four, the low-precision path LP_NKERNEL), then alpha/beta.
+constexpr int NTAB_AB = LP_NKERNEL > 4 ? LP_NKERNEL : 4;
 __global__ __launch_bounds__(GNT) void gk1_kernel(const GOp* ops, const GStage* st, int nstage, int s, const int* pre,
                                                   int nreq)
 {
@@ -261,7 +264,8 @@ struct ngw_batch {
     int next_sys = 1, next_req = 1;
     // grouped launch tables
     DevBuf gops, gstages, gpre, glz;
-    int lp_tot[6] = {0, 0, 0, 0, 0, 0};
+    int lp_tot[LP_NKERNEL] = {};
+    int lp_c1 = 0, lp_c3 = 0;  // tile configurations of the current group
     std::vector<int> active;  // request ids in the current group
     int nstage = 0;
     int t_k1 = 0, t_red = 0, t_k2 = 0, t_k3 = 0, t_ab = 0;
@@ -335,15 +339,15 @@ int launch_any(ngw_batch* b, hipStream_t s, bool capturing)
     if (b->ctx->precision == PREC_F64) {
         NGW_TRY(launch_step_f64(b, s, prof));
     } else {
-        const int* p[6];
-        for (int k = 0; k < 6; ++k) p[k] = pre + k * (nreq + 1);
+        const int* p[LP_NKERNEL];
+        for (int k = 0; k < LP_NKERNEL; ++k) p[k] = pre + k * (nreq + 1);
         LpOp dummy{};
         LpStage dst{};
         for (int stg = 0; stg < b->nstage; ++stg)
-            NGW_TRY(lp2_launch_any(b->ctx->precision, s, b->gops.as<LpOp>(), dummy, b->gstages.as<LpStage>(), dst,
-                                   b->nstage, stg, p, b->lp_tot, nreq, prof));
+            NGW_TRY(lp2_launch_any(b->ctx->precision, s, b->lp_c1, b->lp_c3, b->gops.as<LpOp>(), dummy,
+                                   b->gstages.as<LpStage>(), dst, b->nstage, stg, p, b->lp_tot, nreq, prof));
     }
-    const int* p_ab = pre + 6 * (nreq + 1);
+    const int* p_ab = pre + NTAB_AB * (nreq + 1);
     {
         ProfScope ps(prof, KC_LANCZOS);
         hipLaunchKernelGGL(gab_kernel, dim3(b->t_ab), dim3(GRT), 0, s, b->glz.as<GLz>(), p_ab, nreq);
@@ -371,18 +375,22 @@ int rebuild_group(ngw_batch* b)
     }
     b->nstage = npoly;
     const int prec = b->ctx->precision;
+    if (prec != PREC_F64) {
+        b->lp_c1 = b->ctx->lp_cfg[prec][0];
+        b->lp_c3 = b->ctx->lp_cfg[prec][1];
+    }
     std::vector<GOp> ops(nreq);
     std::vector<GStage> stages((size_t)nreq * npoly);
     std::vector<LpOp> lops(nreq);
     std::vector<LpStage> lstages((size_t)nreq * npoly);
     std::vector<GLz> lz(nreq);
-    std::vector<int> pre(7 * (nreq + 1), 0);
+    std::vector<int> pre((NTAB_AB + 1) * (nreq + 1), 0);
     auto P = [&](int k, int i) -> int& { return pre[(size_t)k * (nreq + 1) + i]; };
     for (int i = 0; i < nreq; ++i) {
         auto& rq = b->reqs[b->active[i]];
         ngw_context* op = b->systems[rq.sys].op.get();
         const long ncv = op->ncv;
-        const int B = rq.B, S = std::max(k1_slots(op->nval), (op->nval - 1) / 64 + 2);
+        const int B = rq.B, S = lp2_slots(op->nval);
         double* vec = rq.vec.as<double>();
         double* q = vec;
         double* w = vec + (size_t)ncv * B;
@@ -422,7 +430,7 @@ int rebuild_group(ngw_batch* b)
             NGW_TRY(lp2_prepare(op));
             NGW_TRY(rq.lpwork.ensure(lp2_work_bytes(prec, nmu, nv, nc, B)));
             LpOp d;
-            lp2_fill(op, prec, B, ncv, rq.tda, rq.lpwork.as<char>(), tpart, u, d);
+            lp2_fill(op, prec, B, ncv, rq.tda, rq.lpwork.as<char>(), tpart, d);
             d.q = q; d.w = w; d.a = z.a; d.b = z.b; d.niter = rq.niter; d.kstep = z.kstep;
             lops[i] = d;
             for (int sgi = 0; sgi < npoly; ++sgi) {
@@ -430,11 +438,11 @@ int rebuild_group(ngw_batch* b)
                 lstages[(size_t)i * npoly + sgi] = LpStage{p.x, p.out, p.y, p.alpha, p.bx, p.by, p.gamma,
                                                            p.accumulate ? 2 : 0};
             }
-            int tot[6];
-            lp2_grid_counts(prec, d, tot);
-            for (int k = 0; k < 6; ++k) P(k, i + 1) = P(k, i) + tot[k];
+            int tot[LP_NKERNEL];
+            lp2_grid_counts(prec, b->lp_c1, b->lp_c3, d, tot);
+            for (int k = 0; k < LP_NKERNEL; ++k) P(k, i + 1) = P(k, i) + tot[k];
         }
-        P(6, i + 1) = P(6, i) + B;
+        P(NTAB_AB, i + 1) = P(NTAB_AB, i) + B;
     }
     hipStream_t s = b->ctx->stream;
     if (prec == PREC_F64) {
@@ -444,14 +452,14 @@ int rebuild_group(ngw_batch* b)
         NGW_HIP(hipMemcpyAsync(b->gops.ptr, ops.data(), sizeof(GOp) * ops.size(), hipMemcpyHostToDevice, s));
         NGW_HIP(hipMemcpyAsync(b->gstages.ptr, stages.data(), sizeof(GStage) * stages.size(), hipMemcpyHostToDevice, s));
     } else {
-        for (int k = 0; k < 6; ++k) b->lp_tot[k] = P(k, nreq);
+        for (int k = 0; k < LP_NKERNEL; ++k) b->lp_tot[k] = P(k, nreq);
         NGW_TRY(b->gops.ensure(sizeof(LpOp) * lops.size()));
         NGW_TRY(b->gstages.ensure(sizeof(LpStage) * lstages.size()));
         NGW_HIP(hipMemcpyAsync(b->gops.ptr, lops.data(), sizeof(LpOp) * lops.size(), hipMemcpyHostToDevice, s));
         NGW_HIP(hipMemcpyAsync(b->gstages.ptr, lstages.data(), sizeof(LpStage) * lstages.size(),
                                hipMemcpyHostToDevice, s));
     }
-    b->t_ab = P(6, nreq);
+    b->t_ab = P(NTAB_AB, nreq);
     NGW_TRY(b->glz.ensure(sizeof(GLz) * lz.size()));
     NGW_TRY(b->gpre.ensure(sizeof(int) * pre.size()));
     NGW_HIP(hipMemcpyAsync(b->glz.ptr, lz.data(), sizeof(GLz) * lz.size(), hipMemcpyHostToDevice, s));
@@ -572,7 +580,7 @@ int ngw_batch_submit(ngw_batch* b, int sys_id, int nqp, const int* qp_band, int 
     const long ncv = op->ncv;
     rq.B = B;
     rq.bytes = request_bytes(op, nqp, niter, nen);
-    const int S = std::max(k1_slots(op->nval), (op->nval - 1) / 64 + 2);
+    const int S = lp2_slots(op->nval);
     int st = NGW_OK;
     auto fail_req = [&](int code) {
         b->reqs.erase(id);

@@ -13,6 +13,8 @@
 
 #include "gemm_core_lowp.h"
 
+#include <type_traits>
+
 namespace ngw {
 
 template <int KIND>
@@ -33,7 +35,9 @@ struct V2Traits<LP_BF16X3> {
     static constexpr int BK = 32, PADV = 8, NSPLIT = 2;
 };
 
-// A pre-converted operand: rows x ld elements (hi part; lo part for BF16x3).
+// A pre-converted operand: rows x ld elements (hi part; lo part for BF16x3),
+// followed by one all-zero row (row `rows`), which tiles read for their rows
+// past the operand, so no masking sits between a load and its LDS store.
 template <int KIND>
 struct OperandV2 {
     const typename V2Traits<KIND>::elem* hi;
@@ -42,140 +46,212 @@ struct OperandV2 {
     int rows;
 };
 
+// LDS slabs.  Rows are 16-byte aligned (row pitch 80 bytes) so operand reads
+// are single ds_read_b128s.
 template <int KIND, int BM, int BN>
-struct SmemV2 {
+struct alignas(16) SmemV2 {
     typedef V2Traits<KIND> Tr;
-    typename Tr::elem a[2][Tr::NSPLIT][BM][Tr::BK + Tr::PADV];
-    typename Tr::elem b[2][Tr::NSPLIT][BN][Tr::BK + Tr::PADV];
+    alignas(16) typename Tr::elem a[2][Tr::NSPLIT][BM][Tr::BK + Tr::PADV];
+    alignas(16) typename Tr::elem b[2][Tr::NSPLIT][BN][Tr::BK + Tr::PADV];
 };
 
-// One operand slab: MN rows x 64 bytes = 4 x 16-byte chunks per row.
+// Loads through global-address-space pointers: global_load instructions count
+// only in vmcnt, so the compiler can wait for an older slab while a newer
+// one is still in flight (generic pointers give flat loads, which also count
+// in lgkmcnt and force full waits).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ T gload(const void* p)
+{
+    return *(const __attribute__((address_space(1))) T*)(p);
+}
+
+// Compile-time loop: f(std::integral_constant<int, 0>) ... f(<N - 1>).
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for_impl(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for_impl<I + 1, N>(f);
+    }
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl<0, N>(f);
+}
+
+// Padding of the operands' k extent (a multiple of every slab depth).
+constexpr int LP_KPAD = 64;
+
+// One operand slab: MN rows x 64 bytes = 4 x 16-byte chunks per row.  The
+// loader keeps addresses; the fetched bytes live in a Raw register set so
+// that several slabs can be in flight (see gemm_v2_run).
 template <int KIND, int MN>
 struct SlabV2 {
     typedef V2Traits<KIND> Tr;
     static constexpr int CH = 64 / 16;             // 16-byte chunks per row
     static constexpr int PER = MN * CH / GNT;      // chunks per thread
     static constexpr int EPC = 16 / sizeof(typename Tr::elem);  // elements per chunk
-    uint4 reg[Tr::NSPLIT][PER];
-    const char* base[Tr::NSPLIT][PER];
-    bool ok[PER];
+    static_assert(PER >= 1 && MN * CH == PER * GNT, "slab split");
+    struct Raw {
+        u32x4 r[Tr::NSPLIT][PER];
+    };
+    const char* base[Tr::NSPLIT];
+    long rowstep;  // bytes between the rows of consecutive chunks of a thread
+    int row0, rows_left;  // rows_left = the zero row, relative to the tile
 
     __device__ __forceinline__ void init(const OperandV2<KIND>& op, int mn0)
     {
         const int t = threadIdx.x;
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int v = t + GNT * i;
-            const int row = mn0 + v / CH, q = v % CH;
-            ok[i] = row < op.rows;
-            const long off = (long)(ok[i] ? row : 0) * op.ld * (long)sizeof(typename Tr::elem) + q * 16;
-            base[0][i] = reinterpret_cast<const char*>(op.hi) + off;
-            if constexpr (Tr::NSPLIT == 2) base[1][i] = reinterpret_cast<const char*>(op.lo) + off;
-        }
+        row0 = t / CH;
+        rows_left = op.rows - mn0;  // local rows < rows_left are inside the operand
+        const long es = (long)sizeof(typename Tr::elem);
+        const long off = (long)mn0 * op.ld * es + (t % CH) * 16;
+        base[0] = reinterpret_cast<const char*>(op.hi) + off;
+        if constexpr (Tr::NSPLIT == 2) base[1] = reinterpret_cast<const char*>(op.lo) + off;
+        rowstep = op.ld * es;
     }
-    __device__ __forceinline__ void fetch(int k0)
+    __device__ __forceinline__ void fetch(int k0, Raw& raw) const
     {
         const long koff = (long)k0 * (long)sizeof(typename Tr::elem);
 #pragma unroll
-        for (int sp = 0; sp < Tr::NSPLIT; ++sp)
+        for (int i = 0; i < PER; ++i) {
+            int row = row0 + (GNT / CH) * i;
+            row = row < rows_left ? row : rows_left;  // rows past the operand read the zero row
 #pragma unroll
-            for (int i = 0; i < PER; ++i) reg[sp][i] = *reinterpret_cast<const uint4*>(base[sp][i] + koff);
+            for (int sp = 0; sp < Tr::NSPLIT; ++sp) raw.r[sp][i] = gload<u32x4>(base[sp] + (long)row * rowstep + koff);
+        }
     }
-    __device__ __forceinline__ void store(typename Tr::elem (*dst)[MN][Tr::BK + Tr::PADV]) const
+    __device__ __forceinline__ void store(const Raw& raw, typename Tr::elem (*dst)[MN][Tr::BK + Tr::PADV]) const
     {
-        const int t = threadIdx.x;
+        const int q = threadIdx.x % CH;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int v = t + GNT * i;
-            const int row = v / CH, q = v % CH;
+            const int row = row0 + (GNT / CH) * i;
 #pragma unroll
-            for (int sp = 0; sp < Tr::NSPLIT; ++sp) {
-                uint4 x = ok[i] ? reg[sp][i] : make_uint4(0u, 0u, 0u, 0u);
-                *reinterpret_cast<uint4*>(&dst[sp][row][q * EPC]) = x;
-            }
+            for (int sp = 0; sp < Tr::NSPLIT; ++sp) *reinterpret_cast<u32x4*>(&dst[sp][row][q * EPC]) = raw.r[sp][i];
         }
     }
 };
 
+// MFMA work on one LDS slab: the wave's (BM/2) x (BN/2) block of the tile.
 template <int KIND, int BM, int BN>
-__device__ __forceinline__ void gemm_v2_tile(const OperandV2<KIND>& A, const OperandV2<KIND>& B, int m0, int n0,
-                                             int Kpad, AccLp<BM, BN>& acc, SmemV2<KIND, BM, BN>& sm)
+__device__ __forceinline__ void mfma_slab(const SmemV2<KIND, BM, BN>& sm, int cur, AccLp<BM, BN>& acc)
 {
     typedef V2Traits<KIND> Tr;
-    constexpr int FM = BM / 32, FN = BN / 32, BK = Tr::BK;
+    constexpr int FM = BM / 32, FN = BN / 32;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
     const int r16 = lane & 15, g = lane >> 4;
+    if constexpr (KIND == LP_F32) {
+        float af[FM][4], bf[FN][4];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            float4 x = *reinterpret_cast<const float4*>(&sm.a[cur][0][wm + 16 * i + r16][4 * g]);
+            af[i][0] = x.x; af[i][1] = x.y; af[i][2] = x.z; af[i][3] = x.w;
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            float4 x = *reinterpret_cast<const float4*>(&sm.b[cur][0][wn + 16 * j + r16][4 * g]);
+            bf[j][0] = x.x; bf[j][1] = x.y; bf[j][2] = x.z; bf[j][3] = x.w;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][s], bf[j][s], acc.v[i][j], 0, 0, 0);
+    } else {
+        bf16x8 ah[FM], bh[FN], alo[Tr::NSPLIT == 2 ? FM : 1], blo[Tr::NSPLIT == 2 ? FN : 1];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            ah[i] = *reinterpret_cast<const bf16x8*>(&sm.a[cur][0][wm + 16 * i + r16][8 * g]);
+            if constexpr (Tr::NSPLIT == 2)
+                alo[i] = *reinterpret_cast<const bf16x8*>(&sm.a[cur][1][wm + 16 * i + r16][8 * g]);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            bh[j] = *reinterpret_cast<const bf16x8*>(&sm.b[cur][0][wn + 16 * j + r16][8 * g]);
+            if constexpr (Tr::NSPLIT == 2)
+                blo[j] = *reinterpret_cast<const bf16x8*>(&sm.b[cur][1][wn + 16 * j + r16][8 * g]);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                if constexpr (Tr::NSPLIT == 2) {
+                    acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[i], bh[j], acc.v[i][j], 0, 0, 0);
+                    acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], blo[j], acc.v[i][j], 0, 0, 0);
+                }
+                acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc.v[i][j], 0, 0, 0);
+            }
+    }
+}
+
+// C tile (BM x BN) = A B^T over Kpad (a multiple of the slab depth).  LA / LB
+// expose fetch(k0, raw) (global -> registers) and store(raw, dst) (registers
+// -> LDS).  D register sets keep D slabs in flight: slab t + D is requested
+// while slab t is multiplied, and slab t + 1 (requested D - 1 slabs earlier)
+// is written to the other LDS buffer; one barrier per slab.  The steady loop
+// runs whole groups of D slabs and is branch-free (requests past the last
+// slab re-read it); the last nslab mod D slabs follow in an unrolled tail.
+template <int KIND, int BM, int BN, int D, class LA, class LB>
+__device__ __forceinline__ void gemm_v2_run(const LA& la, const LB& lb, int Kpad, AccLp<BM, BN>& acc,
+                                            SmemV2<KIND, BM, BN>& sm)
+{
+    typedef V2Traits<KIND> Tr;
+    constexpr int FM = BM / 32, FN = BN / 32, BK = Tr::BK;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc.v[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nslab = Kpad / BK;
+    const int last = (nslab - 1) * BK;
+    typename LA::Raw ra[D];
+    typename LB::Raw rb[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        const int k = min(s * BK, last);
+        la.fetch(k, ra[s]);
+        lb.fetch(k, rb[s]);
+    }
+    la.store(ra[0], sm.a[0]);
+    lb.store(rb[0], sm.b[0]);
+    __syncthreads();
+    // one slab; u = t mod D is a compile-time register-set index
+    auto slab = [&](int t, auto uc) {
+        constexpr int u = decltype(uc)::value;
+        const int k = min((t + D) * BK, last);
+        la.fetch(k, ra[u]);
+        lb.fetch(k, rb[u]);
+        mfma_slab<KIND, BM, BN>(sm, t & 1, acc);
+        la.store(ra[(u + 1) % D], sm.a[(t + 1) & 1]);
+        lb.store(rb[(u + 1) % D], sm.b[(t + 1) & 1]);
+        __syncthreads();
+    };
+    const int nfull = nslab - nslab % D;
+    int t0 = 0;
+    for (; t0 < nfull; t0 += D) {
+        static_for<D>([&](auto uc) { slab(t0 + decltype(uc)::value, uc); });
+    }
+    const int rem = nslab - nfull;
+    static_for<D - 1>([&](auto uc) {
+        if (decltype(uc)::value < rem) slab(t0 + decltype(uc)::value, uc);
+    });
+}
+
+template <int KIND, int BM, int BN, int D = 1>
+__device__ __forceinline__ void gemm_v2_tile(const OperandV2<KIND>& A, const OperandV2<KIND>& B, int m0, int n0,
+                                             int Kpad, AccLp<BM, BN>& acc, SmemV2<KIND, BM, BN>& sm)
+{
     SlabV2<KIND, BM> la;
     SlabV2<KIND, BN> lb;
     la.init(A, m0);
     lb.init(B, n0);
-    const int nslab = Kpad / BK;
-    la.fetch(0);
-    lb.fetch(0);
-    la.store(sm.a[0]);
-    lb.store(sm.b[0]);
-    __syncthreads();
-    for (int t = 0; t < nslab; ++t) {
-        const int cur = t & 1;
-        if (t + 1 < nslab) {
-            la.fetch((t + 1) * BK);
-            lb.fetch((t + 1) * BK);
-        }
-        if constexpr (KIND == LP_F32) {
-            float af[FM][4], bf[FN][4];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                float4 x = *reinterpret_cast<const float4*>(&sm.a[cur][0][wm + 16 * i + r16][4 * g]);
-                af[i][0] = x.x; af[i][1] = x.y; af[i][2] = x.z; af[i][3] = x.w;
-            }
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                float4 x = *reinterpret_cast<const float4*>(&sm.b[cur][0][wn + 16 * j + r16][4 * g]);
-                bf[j][0] = x.x; bf[j][1] = x.y; bf[j][2] = x.z; bf[j][3] = x.w;
-            }
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-#pragma unroll
-                for (int i = 0; i < FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < FN; ++j)
-                        acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][s], bf[j][s], acc.v[i][j], 0, 0, 0);
-        } else {
-            bf16x8 ah[FM], bh[FN], alo[Tr::NSPLIT == 2 ? FM : 1], blo[Tr::NSPLIT == 2 ? FN : 1];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                ah[i] = *reinterpret_cast<const bf16x8*>(&sm.a[cur][0][wm + 16 * i + r16][8 * g]);
-                if constexpr (Tr::NSPLIT == 2)
-                    alo[i] = *reinterpret_cast<const bf16x8*>(&sm.a[cur][1][wm + 16 * i + r16][8 * g]);
-            }
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                bh[j] = *reinterpret_cast<const bf16x8*>(&sm.b[cur][0][wn + 16 * j + r16][8 * g]);
-                if constexpr (Tr::NSPLIT == 2)
-                    blo[j] = *reinterpret_cast<const bf16x8*>(&sm.b[cur][1][wn + 16 * j + r16][8 * g]);
-            }
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    if constexpr (Tr::NSPLIT == 2) {
-                        acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[i], bh[j], acc.v[i][j], 0, 0, 0);
-                        acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], blo[j], acc.v[i][j], 0, 0, 0);
-                    }
-                    acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc.v[i][j], 0, 0, 0);
-                }
-        }
-        if (t + 1 < nslab) {
-            la.store(sm.a[cur ^ 1]);
-            lb.store(sm.b[cur ^ 1]);
-        }
-        __syncthreads();
-    }
+    gemm_v2_run<KIND, BM, BN, D>(la, lb, Kpad, acc, sm);
 }
 
 // FP32 value -> stored element(s) of a pre-converted operand.

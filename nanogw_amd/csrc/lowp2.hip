@@ -6,14 +6,17 @@
 //   per system (once):  pct = (Psi_c / sigma_c)^T  [mu][c]   K1 A operand
 //                       pc  =  Psi_c / sigma_c     [c][mu]   K3 A operand
 //                       m32t = M^T FP32            [mu][nu]  K2 A operand
+//                       psiv32 = Psi_v^T FP32      [v][mu]   K1 epilogue, K3 B factor
 //   per operator application:
 //     xprep  x'(j, c) = sigma_c s(v,c) x_b(v,c)   (transpose through LDS)
 //     K1     T = pct x'^T, epilogue * Psi_v and reduce over v (LDS)
 //     red    t(b, nu) = sum of K1 slots  -> FP32 K2 B operand
-//     K2     u = M t (FP32 MFMA)
-//     wprep  W(j, mu) = Psi_v(mu, v) u(mu, b)
+//     K2     u = M t (FP32 MFMA), stored FP32 [b][mu]
+//     wprep  W(j, mu) = Psi_v(mu, v) u(mu, b)  (8 elements per thread)
 //     K3     Y = pc W^T, epilogue * sigma_c, then the FP64 combination
 //            (diagonal term, Horner/Clenshaw stage, Lanczos accumulate).
+// All [row][k] operand arrays have k padded with zeros to a multiple of
+// LP_KPAD and one extra all-zero row after the last (see OperandV2).
 // Every kernel reads one request descriptor (by value for a single context,
 // from a device table for the grouped batch engine), so molecules of
 // different sizes share a launch; tiles never mix requests.
@@ -38,14 +41,26 @@ __device__ __forceinline__ int find_pre(const int* pre, int nreq, int bid)
     return lo;
 }
 
-template <int KIND>
-struct K1Tile {
-    static constexpr int BM = 128, BN = (KIND == LP_BF16X3) ? 64 : 128;
-};
-constexpr int L2K3M = 64, L2K3N = 128, L2K2 = 64;
+constexpr int L2K2 = 64;
 
-// K1 tile width for a precision mode (NGW_PREC_* / Precision value).
-static int k1_bn_of_prec(int prec) { return prec == PREC_BF16X3 ? K1Tile<LP_BF16X3>::BN : K1Tile<LP_BF16>::BN; }
+// Tile configurations of the K1 and K3 GEMMs: {BM, BN, D} with D the number
+// of slabs in flight (register prefetch depth).  Index 0 is the default; the
+// autotuner (ngw_autotune) times all of them on the caller's shape.
+struct LpCfg {
+    int bm, bn, d;
+};
+constexpr int LP_NCFG = 4;
+constexpr LpCfg K1_CFG[2][LP_NCFG] = {
+    {{128, 128, 2}, {128, 128, 3}, {256, 128, 2}, {128, 64, 2}},    // FP32, BF16
+    {{128, 128, 2}, {128, 64, 2}, {128, 128, 1}, {128, 64, 3}}};    // BF16x3
+constexpr LpCfg K3_CFG[2][LP_NCFG] = {
+    {{128, 128, 2}, {128, 128, 3}, {64, 128, 2}, {64, 128, 3}},
+    {{128, 128, 2}, {64, 128, 2}, {128, 128, 1}, {64, 128, 3}}};
+__host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : 0; }
+static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : 0; }
+
+// Narrowest K1 tile of any configuration: sizes the slot partial sums.
+constexpr int K1_BN_MIN = 64;
 
 // ---------------------------------------------------------------- per-system preparation
 template <int KIND>
@@ -53,16 +68,16 @@ __global__ void prep_psi_kernel(const double* psi, long ldp, int nmu, int nv, in
                                 int ldm1, void* pct_hi, void* pct_lo, void* pc_hi, void* pc_lo)
 {
     typedef typename V2Traits<KIND>::elem E;
-    const long total = (long)nmu * ldc1 + (long)nc * ldm1;
+    const long n1 = (long)(nmu + 1) * ldc1, total = n1 + (long)(nc + 1) * ldm1;  // + zero rows
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-        if (t < (long)nmu * ldc1) {
+        if (t < n1) {
             const int mu = (int)(t / ldc1), c = (int)(t - (long)mu * ldc1);
-            float v = c < nc ? (float)(psi[mu + (long)(nv + c) * ldp] / (double)sig[nv + c]) : 0.0f;
+            float v = (c < nc && mu < nmu) ? (float)(psi[mu + (long)(nv + c) * ldp] / (double)sig[nv + c]) : 0.0f;
             put_v2<KIND>((E*)pct_hi, (E*)pct_lo, t, v);
         } else {
-            const long u = t - (long)nmu * ldc1;
+            const long u = t - n1;
             const int c = (int)(u / ldm1), mu = (int)(u - (long)c * ldm1);
-            float v = mu < nmu ? (float)(psi[mu + (long)(nv + c) * ldp] / (double)sig[nv + c]) : 0.0f;
+            float v = (mu < nmu && c < nc) ? (float)(psi[mu + (long)(nv + c) * ldp] / (double)sig[nv + c]) : 0.0f;
             put_v2<KIND>((E*)pc_hi, (E*)pc_lo, u, v);
         }
     }
@@ -71,35 +86,35 @@ __global__ void prep_psi_kernel(const double* psi, long ldp, int nmu, int nv, in
 __global__ void prep_m_kernel(const double* m, long ldm, int nmu, int ldm1, const double* psi, long ldp, int nv,
                               float* m32t, float* psiv32)
 {
-    const long total = (long)nmu * ldm1 + (long)nmu * nv;
+    const long n1 = (long)(nmu + 1) * ldm1, total = n1 + (long)nv * ldm1;  // m32t + zero row, psiv32
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-        if (t < (long)nmu * ldm1) {
+        if (t < n1) {
             const int mu = (int)(t / ldm1), nu = (int)(t - (long)mu * ldm1);
-            m32t[t] = nu < nmu ? (float)m[mu + (long)nu * ldm] : 0.0f;  // row mu of M, k = nu
+            m32t[t] = (nu < nmu && mu < nmu) ? (float)m[mu + (long)nu * ldm] : 0.0f;  // row mu of M, k = nu
         } else {
-            const long u = t - (long)nmu * ldm1;
-            const int v = (int)(u / nmu), mu = (int)(u - (long)v * nmu);
-            psiv32[u] = (float)psi[mu + (long)v * ldp];
+            const long u = t - n1;
+            const int v = (int)(u / ldm1), mu = (int)(u - (long)v * ldm1);
+            psiv32[u] = mu < nmu ? (float)psi[mu + (long)v * ldp] : 0.0f;
         }
     }
 }
 
-static int pad32(int n) { return (n + 31) & ~31; }
+static int padk(int n) { return (n + LP_KPAD - 1) / LP_KPAD * LP_KPAD; }
 
 int lp2_prepare(ngw_context* ctx)
 {
     if (ctx->lp2_ready) return NGW_OK;
     NGW_TRY(lowp_prepare(ctx));  // per-band scales
     const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond;
-    ctx->ldc1 = pad32(nc);
-    ctx->ldm1 = pad32(nmu);
-    const size_t n1 = (size_t)nmu * ctx->ldc1, n2 = (size_t)nc * ctx->ldm1;
+    ctx->ldc1 = padk(nc);
+    ctx->ldm1 = padk(nmu);
+    const size_t n1 = (size_t)(nmu + 1) * ctx->ldc1, n2 = (size_t)(nc + 1) * ctx->ldm1;  // + zero rows
     NGW_TRY(ctx->pct32.ensure(4 * n1));
     NGW_TRY(ctx->pc32.ensure(4 * n2));
     NGW_TRY(ctx->pctb.ensure(2 * 2 * n1));  // hi then lo
     NGW_TRY(ctx->pcb.ensure(2 * 2 * n2));
-    NGW_TRY(ctx->m32t.ensure(4 * (size_t)nmu * ctx->ldm1));
-    NGW_TRY(ctx->psiv32.ensure(4 * (size_t)nmu * nv));
+    NGW_TRY(ctx->m32t.ensure(4 * (size_t)(nmu + 1) * ctx->ldm1));
+    NGW_TRY(ctx->psiv32.ensure(4 * (size_t)nv * ctx->ldm1));
     hipStream_t s = ctx->stream;
     const double* psi = ctx->psi.as<double>();
     const float* sig = ctx->psi_scale.as<float>();
@@ -122,67 +137,86 @@ int lp2_prepare(ngw_context* ctx)
     const LpOp op = ops ? ops[r] : op1;                                  \
     const int local = pre ? (int)blockIdx.x - pre[r] : (int)blockIdx.x;
 
+// GEMM kernels: the workgroup index is first mapped so that each XCD owns a
+// contiguous range of tiles (M fastest inside a request), then to a request.
+#define LP_PICK_TILE()                                                   \
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);           \
+    const int r = find_pre(pre, nreq, bid);                              \
+    const LpOp op = ops ? ops[r] : op1;                                  \
+    const int local = pre ? bid - pre[r] : bid;
+
 // ---------------------------------------------------------------- conversion passes
-// x'(j = v + nv b, c) = sigma_c s(v,c) x_b(v,c); 32 x 32 (v, c) tiles through LDS.
+// x'(j = v + nv b, c) = sigma_c s(v,c) x_b(v,c) in (32 v) x (64 c) tiles
+// through LDS: reads run along v (contiguous in x), writes are eight
+// consecutive c per thread in one 16- or 32-byte store.
 template <int KIND>
 __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                     int nstage, int s, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
-    __shared__ float tile[32][33];
+    __shared__ float tile[64][33];
     LP_PICK_OP();
     const double* x = ops ? st[r * nstage + s].x : st1.x;
-    const int nvt = (op.nv + 31) / 32, nct = op.ldc1 / 32;
+    const int nvt = (op.nv + 31) / 32, nct = op.ldc1 / 64;
     const int b = local / (nvt * nct);
     const int rem = local - b * nvt * nct;
-    const int v0 = (rem / nct) * 32, c0 = (rem % nct) * 32;
+    const int v0 = (rem / nct) * 32, c0 = (rem % nct) * 64;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    if (local == 0)  // the zero row after the last x' row
+        for (int c = threadIdx.x; c < op.ldc1; c += 256)
+            put_v2<KIND>((E*)op.xlp_hi, (E*)op.xlp_lo, (long)op.nv * op.nvec * op.ldc1 + c, 0.0f);
+    const double* xb = x + (long)b * op.ld;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 8; ++i) {
         const int v = v0 + tx, c = c0 + ty + 8 * i;
         float val = 0.0f;
         if (v < op.nv && c < op.nc) {
             const long idx = v + (long)c * op.nv;
-            double xv = x[idx + (long)b * op.ld];
-            double xs = op.tda ? xv : op.sqrtr[idx] * xv;
+            const double xv = xb[idx];
+            const double xs = op.tda ? xv : op.sqrtr[idx] * xv;
             val = (float)(xs * (double)op.sig[op.nv + c]);
         }
         tile[ty + 8 * i][tx] = val;
     }
     __syncthreads();
+    const int v = v0 + (threadIdx.x >> 3), cq = (threadIdx.x & 7) * 8;
+    if (v >= op.nv) return;
+    float w[8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int c = c0 + tx, v = v0 + ty + 8 * i;
-        if (v < op.nv)
-            put_v2<KIND>((E*)op.xlp_hi, (E*)op.xlp_lo, (long)(v + op.nv * b) * op.ldc1 + c, tile[tx][ty + 8 * i]);
+    for (int e = 0; e < 8; ++e) w[e] = tile[cq + e][threadIdx.x >> 3];
+    const long o = (long)(v + op.nv * b) * op.ldc1 + c0 + cq;
+    if constexpr (KIND == LP_F32) {
+        f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.xlp_hi) + o);
+        out[0] = f32x4{w[0], w[1], w[2], w[3]};
+        out[1] = f32x4{w[4], w[5], w[6], w[7]};
+    } else {
+        unsigned hi[4], lo[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const unsigned short ha = f32_to_bf16_bits(w[2 * e]), hb = f32_to_bf16_bits(w[2 * e + 1]);
+            hi[e] = (unsigned)ha | ((unsigned)hb << 16);
+            if constexpr (V2Traits<KIND>::NSPLIT == 2) {
+                const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
+                const unsigned short lb = f32_to_bf16_bits(w[2 * e + 1] - bf16_bits_to_f32(hb));
+                lo[e] = (unsigned)la | ((unsigned)lb << 16);
+            }
+        }
+        *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.xlp_hi) + o) = u32x4{hi[0], hi[1], hi[2], hi[3]};
+        if constexpr (V2Traits<KIND>::NSPLIT == 2)
+            *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.xlp_lo) + o) = u32x4{lo[0], lo[1], lo[2], lo[3]};
     }
 }
 
-// W(j = v + nv b, mu) = Psi_v(mu, v) u(mu, b)
-template <int KIND>
-__global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
-{
-    typedef typename V2Traits<KIND>::elem E;
-    LP_PICK_OP();
-    const int per_row = op.ldm1 / 256 + ((op.ldm1 % 256) ? 1 : 0);
-    const int j = local / per_row;
-    const int mu = (local - j * per_row) * 256 + threadIdx.x;
-    if (mu >= op.ldm1) return;
-    const int b = j / op.nv, v = j - b * op.nv;
-    float val = 0.0f;
-    if (mu < op.nmu) val = (float)(op.psi[mu + (long)v * op.ldp] * op.u[mu + (long)b * op.nmu]);
-    put_v2<KIND>((E*)op.wlp_hi, (E*)op.wlp_lo, (long)j * op.ldm1 + mu, val);
-}
-
-// t32(b, nu) = sum of the K1 slots covering vector b (fixed order); zero padding.
+// t32(b, nu) = sum of the K1 slots covering vector b (fixed order); zero
+// padding and zero row b = nvec.
 __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq, int bn)
 {
     LP_PICK_OP();
     const long i = (long)local * 256 + threadIdx.x;
-    if (i >= (long)op.nvec * op.ldm1) return;
+    if (i >= (long)(op.nvec + 1) * op.ldm1) return;
     const int b = (int)(i / op.ldm1), nu = (int)(i - (long)b * op.ldm1);
     float out = 0.0f;
-    if (nu < op.nmu) {
+    if (nu < op.nmu && b < op.nvec) {
         const int first = (b * op.nv) / bn, last = ((b + 1) * op.nv - 1) / bn;
         double acc = 0.0;
         for (int sl = 0; sl <= last - first; ++sl) acc += op.tpart[((long)sl * op.nvec + b) * op.nmu + nu];
@@ -192,75 +226,125 @@ __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, co
 }
 
 // ---------------------------------------------------------------- GEMMs
-template <int KIND>
+template <int KIND, int BM, int BN, int D>
 __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
-    constexpr int BM = K1Tile<KIND>::BM, BN = K1Tile<KIND>::BN;
     typedef typename V2Traits<KIND>::elem E;
+    constexpr int HALF = BM < 64 ? BM : 64;  // rows reduced per LDS pass (keeps the buffer within the slabs' LDS)
     __shared__ union {
         SmemV2<KIND, BM, BN> ab;
-        float red[BM][BN + 1];
+        float red[HALF][BN + 1];
     } sm;
-    LP_PICK_OP();
+    LP_PICK_TILE();
     const int nv = op.nv, N = nv * op.nvec;
     const int mt = (op.nmu + BM - 1) / BM;
     const int tm = local % mt, tn = local / mt;
     const int m0 = tm * BM, n0 = tn * BN;
     AccLp<BM, BN> acc;
-    gemm_v2_tile<KIND, BM, BN>(OperandV2<KIND>{(const E*)op.pct_hi, (const E*)op.pct_lo, op.ldc1, op.nmu},
-                               OperandV2<KIND>{(const E*)op.xlp_hi, (const E*)op.xlp_lo, op.ldc1, N}, m0, n0, op.ldc1,
-                               acc, sm.ab);
-    for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
-        int gm = m0 + row, gn = n0 + col;
-        float p = 0.0f;
-        if (gm < op.nmu && gn < N) p = val * op.psiv32[gm + (long)(gn % nv) * op.nmu];
-        sm.red[row][col] = p;
-    });
-    __syncthreads();
+    gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pct_hi, (const E*)op.pct_lo, op.ldc1, op.nmu},
+                                  OperandV2<KIND>{(const E*)op.xlp_hi, (const E*)op.xlp_lo, op.ldc1, N}, m0, n0,
+                                  op.ldc1, acc, sm.ab);
     const int nend = min(n0 + BN, N);
     const int b_lo = n0 / nv, b_hi = (nend - 1) / nv;
     const int nbt = b_hi - b_lo + 1;
-    for (int e = threadIdx.x; e < BM * nbt; e += GNT) {
-        int row = e % BM, b = b_lo + e / BM;
-        int c0 = max(n0, b * nv) - n0, c1 = min(nend, (b + 1) * nv) - n0;
-        float s = 0.0f;
-        for (int cc = c0; cc < c1; ++cc) s += sm.red[row][cc];
-        int gm = m0 + row;
-        int slot = tn - (b * nv) / BN;
-        if (gm < op.nmu) op.tpart[((long)slot * op.nvec + b) * op.nmu + gm] = (double)s;
+#pragma unroll
+    for (int pass = 0; pass < BM / HALF; ++pass) {
+        if (pass > 0) __syncthreads();
+        for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
+            if (row < pass * HALF || row >= (pass + 1) * HALF) return;
+            int gm = m0 + row, gn = n0 + col;
+            float p = 0.0f;
+            if (gm < op.nmu && gn < N) p = val * op.psiv32[gm + (long)(gn % nv) * op.ldm1];
+            sm.red[row - pass * HALF][col] = p;
+        });
+        __syncthreads();
+        for (int e = threadIdx.x; e < HALF * nbt; e += GNT) {
+            int row = e % HALF, b = b_lo + e / HALF;
+            int c0 = max(n0, b * nv) - n0, c1 = min(nend, (b + 1) * nv) - n0;
+            float sum = 0.0f;
+            for (int cc = c0; cc < c1; ++cc) sum += sm.red[row][cc];
+            int gm = m0 + pass * HALF + row;
+            int slot = tn - (b * nv) / BN;
+            if (gm < op.nmu) op.tpart[((long)slot * op.nvec + b) * op.nmu + gm] = (double)sum;
+        }
     }
 }
 
 __global__ __launch_bounds__(GNT) void k2v2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
     __shared__ SmemV2<LP_F32, L2K2, L2K2> sm;
-    LP_PICK_OP();
+    LP_PICK_TILE();
     const int mt = (op.nmu + L2K2 - 1) / L2K2;
     const int m0 = (local % mt) * L2K2, n0 = (local / mt) * L2K2;
     AccLp<L2K2, L2K2> acc;
     gemm_v2_tile<LP_F32, L2K2, L2K2>(OperandV2<LP_F32>{op.m32t, nullptr, op.ldm1, op.nmu},
                                      OperandV2<LP_F32>{op.t32, nullptr, op.ldm1, op.nvec}, m0, n0, op.ldm1, acc, sm);
+    // rows nmu <= mu < ldm1 come out as exact zeros (zero A rows): the K3
+    // loader reads u over the whole padded slab
     for_acc_lp<L2K2, L2K2>(acc, [&](int row, int col, float val) {
         int gm = m0 + row, gn = n0 + col;
-        if (gm < op.nmu && gn < op.nvec) op.u[gm + (long)gn * op.nmu] = (double)val;
+        if (gm < op.ldm1 && gn < op.nvec) op.u32[gm + (long)gn * op.ldm1] = val;
     });
 }
 
+// W(j = v + nv b, mu) = Psi_v(mu, v) u(mu, b) for rows j <= N (row N zero),
+// eight consecutive mu per thread from the FP32 psiv32 / u rows.
 template <int KIND>
+__global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
+{
+    typedef typename V2Traits<KIND>::elem E;
+    LP_PICK_OP();
+    const int per_row = op.ldm1 / 8, N = op.nv * op.nvec;
+    const long i = (long)local * 256 + threadIdx.x;
+    if (i >= (long)(N + 1) * per_row) return;
+    const int j = (int)(i / per_row), mu = (int)(i - (long)j * per_row) * 8;
+    f32x4 w0 = {0.f, 0.f, 0.f, 0.f}, w1 = w0;
+    if (j < N) {
+        const int b = j / op.nv, v = j - b * op.nv;
+        const float* p = op.psiv32 + (long)v * op.ldm1 + mu;
+        const float* u = op.u32 + (long)b * op.ldm1 + mu;
+        w0 = gload<f32x4>(p) * gload<f32x4>(u);
+        w1 = gload<f32x4>(p + 4) * gload<f32x4>(u + 4);
+    }
+    const long o = (long)j * op.ldm1 + mu;
+    if constexpr (KIND == LP_F32) {
+        f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.wlp_hi) + o);
+        out[0] = w0;
+        out[1] = w1;
+    } else {
+        const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+        unsigned hi[4], lo[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const unsigned short ha = f32_to_bf16_bits(w[2 * e]), hb = f32_to_bf16_bits(w[2 * e + 1]);
+            hi[e] = (unsigned)ha | ((unsigned)hb << 16);
+            if constexpr (V2Traits<KIND>::NSPLIT == 2) {
+                const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
+                const unsigned short lb = f32_to_bf16_bits(w[2 * e + 1] - bf16_bits_to_f32(hb));
+                lo[e] = (unsigned)la | ((unsigned)lb << 16);
+            }
+        }
+        *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.wlp_hi) + o) = u32x4{hi[0], hi[1], hi[2], hi[3]};
+        if constexpr (V2Traits<KIND>::NSPLIT == 2)
+            *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.wlp_lo) + o) = u32x4{lo[0], lo[1], lo[2], lo[3]};
+    }
+}
+
+template <int KIND, int BM, int BN, int D>
 __global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                    int nstage, int s, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
-    __shared__ SmemV2<KIND, L2K3M, L2K3N> sm;
-    LP_PICK_OP();
+    __shared__ SmemV2<KIND, BM, BN> sm;
+    LP_PICK_TILE();
     const LpStage sg = ops ? st[r * nstage + s] : st1;
     const int nv = op.nv, N = nv * op.nvec;
-    const int mt = (op.nc + L2K3M - 1) / L2K3M;
-    const int m0 = (local % mt) * L2K3M, n0 = (local / mt) * L2K3N;
-    AccLp<L2K3M, L2K3N> acc;
-    gemm_v2_tile<KIND, L2K3M, L2K3N>(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc},
-                                     OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0, n0,
-                                     op.ldm1, acc, sm);
+    const int mt = (op.nc + BM - 1) / BM;
+    const int m0 = (local % mt) * BM, n0 = (local / mt) * BN;
+    AccLp<BM, BN> acc;
+    gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc},
+                                  OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0, n0,
+                                  op.ldm1, acc, sm);
     OperatorEpilogue ep;
     ep.x = sg.x; ep.ldx = op.ld;
     ep.d = op.tda ? op.dtda : op.dfull;
@@ -274,7 +358,7 @@ __global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, co
     ep.acc = acc_on ? sg.out : nullptr; ep.ldacc = op.ld;
     ep.out = sg.out; ep.ldo = op.ld;
     const float* sigc = op.sig + nv;
-    for_acc_lp<L2K3M, L2K3N>(acc, [&](int row, int col, float val) {
+    for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
         int c = m0 + row, j = n0 + col;
         if (c < op.nc && j < N) {
             int b = j / nv, v = j - b * nv;
@@ -289,22 +373,39 @@ struct LpGrid {
     int xprep, k1, red, k2, wprep, k3;
 };
 
-static LpGrid lp_grid(int kind, int nmu, int nv, int nc, int nvec, int ldc1, int ldm1)
+static LpGrid lp_grid(int prec, int c1, int c3, int nmu, int nv, int nc, int nvec, int ldc1, int ldm1)
 {
+    const LpCfg k1 = K1_CFG[cfg_set_prec(prec)][c1], k3 = K3_CFG[cfg_set_prec(prec)][c3];
     LpGrid g;
-    const int bn = k1_bn_of_prec(kind);
-    g.xprep = nvec * ((nv + 31) / 32) * (ldc1 / 32);
-    g.k1 = ((nmu + 127) / 128) * ((nv * nvec + bn - 1) / bn);
-    g.red = (int)(((long)nvec * ldm1 + 255) / 256);
+    g.xprep = nvec * ((nv + 31) / 32) * (ldc1 / 64);
+    g.k1 = ((nmu + k1.bm - 1) / k1.bm) * ((nv * nvec + k1.bn - 1) / k1.bn);
+    g.red = (int)(((long)(nvec + 1) * ldm1 + 255) / 256);
     g.k2 = ((nmu + L2K2 - 1) / L2K2) * ((nvec + L2K2 - 1) / L2K2);
-    g.wprep = nv * nvec * (ldm1 / 256 + ((ldm1 % 256) ? 1 : 0));
-    g.k3 = ((nc + L2K3M - 1) / L2K3M) * ((nv * nvec + L2K3N - 1) / L2K3N);
+    g.wprep = (int)(((long)(nv * nvec + 1) * (ldm1 / 8) + 255) / 256);
+    g.k3 = ((nc + k3.bm - 1) / k3.bm) * ((nv * nvec + k3.bn - 1) / k3.bn);
     return g;
 }
 
+template <int KIND, int C>
+static void launch_k1(hipStream_t st, int nblk, const LpOp* ops, const LpOp& op1, const int* pre, int nreq)
+{
+    constexpr LpCfg c = K1_CFG[cfg_set(KIND)][C];
+    hipLaunchKernelGGL((k1v2_kernel<KIND, c.bm, c.bn, c.d>), dim3(nblk), dim3(GNT), 0, st, ops, op1, pre, nreq);
+}
+
+template <int KIND, int C>
+static void launch_k3(hipStream_t st, int nblk, const LpOp* ops, const LpOp& op1, const LpStage* stg,
+                      const LpStage& st1, int nstage, int s, const int* pre, int nreq)
+{
+    constexpr LpCfg c = K3_CFG[cfg_set(KIND)][C];
+    hipLaunchKernelGGL((k3v2_kernel<KIND, c.bm, c.bn, c.d>), dim3(nblk), dim3(GNT), 0, st, ops, op1, stg, st1,
+                       nstage, s, pre, nreq);
+}
+
 template <int KIND>
-static int lp_launch(hipStream_t st, const LpOp* ops, const LpOp& op1, const LpStage* stg, const LpStage& st1,
-                     int nstage, int s, const int* const* pre, const int* tot, int nreq, ngw_context* prof)
+static int lp_launch(hipStream_t st, int c1, int c3, const LpOp* ops, const LpOp& op1, const LpStage* stg,
+                     const LpStage& st1, int nstage, int s, const int* const* pre, const int* tot, int nreq,
+                     ngw_context* prof)
 {
     {
         ProfScope ps(prof, KC_CONVERT);
@@ -313,11 +414,17 @@ static int lp_launch(hipStream_t st, const LpOp* ops, const LpOp& op1, const LpS
     }
     {
         ProfScope ps(prof, KC_K1);
-        hipLaunchKernelGGL(k1v2_kernel<KIND>, dim3(tot[1]), dim3(GNT), 0, st, ops, op1, pre[1], nreq);
+        switch (c1) {
+            case 0: launch_k1<KIND, 0>(st, tot[1], ops, op1, pre[1], nreq); break;
+            case 1: launch_k1<KIND, 1>(st, tot[1], ops, op1, pre[1], nreq); break;
+            case 2: launch_k1<KIND, 2>(st, tot[1], ops, op1, pre[1], nreq); break;
+            default: launch_k1<KIND, 3>(st, tot[1], ops, op1, pre[1], nreq); break;
+        }
     }
     {
         ProfScope ps(prof, KC_CONVERT);
-        hipLaunchKernelGGL(red2_kernel, dim3(tot[2]), dim3(256), 0, st, ops, op1, pre[2], nreq, K1Tile<KIND>::BN);
+        hipLaunchKernelGGL(red2_kernel, dim3(tot[2]), dim3(256), 0, st, ops, op1, pre[2], nreq,
+                           K1_CFG[cfg_set(KIND)][c1].bn);
     }
     {
         ProfScope ps(prof, KC_K2);
@@ -329,38 +436,52 @@ static int lp_launch(hipStream_t st, const LpOp* ops, const LpOp& op1, const LpS
     }
     {
         ProfScope ps(prof, KC_K3);
-        hipLaunchKernelGGL(k3v2_kernel<KIND>, dim3(tot[5]), dim3(GNT), 0, st, ops, op1, stg, st1, nstage, s, pre[5],
-                           nreq);
+        switch (c3) {
+            case 0: launch_k3<KIND, 0>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
+            case 1: launch_k3<KIND, 1>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
+            case 2: launch_k3<KIND, 2>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
+            default: launch_k3<KIND, 3>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
+        }
     }
     NGW_HIP(hipGetLastError());
     return NGW_OK;
 }
 
-int lp2_launch_any(int kind, hipStream_t st, const LpOp* ops, const LpOp& op1, const LpStage* stg, const LpStage& st1,
-                   int nstage, int s, const int* const* pre, const int* tot, int nreq, ngw_context* prof)
+int lp2_launch_any(int kind, hipStream_t st, int c1, int c3, const LpOp* ops, const LpOp& op1, const LpStage* stg,
+                   const LpStage& st1, int nstage, int s, const int* const* pre, const int* tot, int nreq,
+                   ngw_context* prof)
 {
+    if (c1 < 0 || c1 >= LP_NCFG || c3 < 0 || c3 >= LP_NCFG) return fail(NGW_ERR_ARG, "tile configuration index");
     switch (kind) {
-        case PREC_F32: return lp_launch<LP_F32>(st, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
-        case PREC_BF16: return lp_launch<LP_BF16>(st, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
-        case PREC_BF16X3: return lp_launch<LP_BF16X3>(st, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
+        case PREC_F32: return lp_launch<LP_F32>(st, c1, c3, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
+        case PREC_BF16: return lp_launch<LP_BF16>(st, c1, c3, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
+        case PREC_BF16X3:
+            return lp_launch<LP_BF16X3>(st, c1, c3, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
     }
     return fail(NGW_ERR_STATE, "unknown precision");
 }
 
-// Bytes of the per-request low-precision workspaces.
+// K1 partial-sum slots per vector for any tile configuration.
+int lp2_slots(int nv) { return std::max(k1_slots(nv), (nv - 1) / K1_BN_MIN + 2); }
+
+// Bytes of the per-request low-precision workspaces: x' ([j][c]) and W
+// ([j][mu]) in the MFMA type, t and u (FP32, [b][mu]); operand arrays carry
+// their zero row; each part starts 256-byte aligned.
+static size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
 size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec)
 {
     const size_t es = kind == PREC_F32 ? 4 : 2, ns = kind == PREC_BF16X3 ? 2 : 1;
-    const size_t ldc1 = pad32(nc), ldm1 = pad32(nmu), N = (size_t)nv * nvec;
-    return es * ns * N * (ldc1 + ldm1) + 4 * (size_t)nvec * ldm1 + 256 * 4;
+    const size_t ldc1 = padk(nc), ldm1 = padk(nmu), N = (size_t)nv * nvec;
+    return ns * (align256(es * (N + 1) * ldc1) + align256(es * (N + 1) * ldm1)) +
+           align256(4 * (size_t)(nvec + 1) * ldm1) + align256(4 * (size_t)nvec * ldm1);
 }
 
 // Fill the descriptor of one request from its operator and workspaces.
-void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char* work, double* tpart, double* u,
-              LpOp& d)
+void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char* work, double* tpart, LpOp& d)
 {
     const size_t es = kind == PREC_F32 ? 4 : 2;
-    const size_t n1 = (size_t)op->n_intp * op->ldc1, n2 = (size_t)op->ncond * op->ldm1;
+    const size_t n1 = (size_t)(op->n_intp + 1) * op->ldc1, n2 = (size_t)(op->ncond + 1) * op->ldm1;
     if (kind == PREC_F32) {
         d.pct_hi = op->pct32.ptr; d.pct_lo = nullptr;
         d.pc_hi = op->pc32.ptr; d.pc_lo = nullptr;
@@ -375,22 +496,24 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
     d.nmu = op->n_intp; d.nv = op->nval; d.nc = op->ncond; d.nvec = nvec; d.ldc1 = op->ldc1; d.ldm1 = op->ldm1;
     d.ld = ld; d.tda = tda;
     const size_t N = (size_t)op->nval * nvec;
-    const size_t xsz = es * N * op->ldc1, wsz = es * N * op->ldm1;
+    const size_t ns = kind == PREC_BF16X3 ? 2 : 1;
+    const size_t xsz = align256(es * (N + 1) * op->ldc1), wsz = align256(es * (N + 1) * op->ldm1);
+    const size_t tsz = align256(4 * (size_t)(nvec + 1) * op->ldm1);
     d.xlp_hi = work;
-    d.xlp_lo = kind == PREC_BF16X3 ? work + xsz : nullptr;
-    char* w0 = work + xsz * (kind == PREC_BF16X3 ? 2 : 1);
+    d.xlp_lo = ns == 2 ? work + xsz : nullptr;
+    char* w0 = work + xsz * ns;
     d.wlp_hi = w0;
-    d.wlp_lo = kind == PREC_BF16X3 ? w0 + wsz : nullptr;
-    char* t0 = w0 + wsz * (kind == PREC_BF16X3 ? 2 : 1);
-    t0 = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(t0) + 255) & ~uintptr_t(255));
+    d.wlp_lo = ns == 2 ? w0 + wsz : nullptr;
+    char* t0 = w0 + wsz * ns;
     d.t32 = reinterpret_cast<float*>(t0);
-    d.tpart = tpart; d.u = u;
+    d.u32 = reinterpret_cast<float*>(t0 + tsz);
+    d.tpart = tpart;
     d.q = nullptr; d.w = nullptr; d.a = nullptr; d.b = nullptr; d.niter = 0; d.kstep = nullptr;
 }
 
-void lp2_grid_counts(int kind, const LpOp& d, int* tot)
+void lp2_grid_counts(int kind, int c1, int c3, const LpOp& d, int* tot)
 {
-    LpGrid g = lp_grid(kind, d.nmu, d.nv, d.nc, d.nvec, d.ldc1, d.ldm1);
+    LpGrid g = lp_grid(kind, c1, c3, d.nmu, d.nv, d.nc, d.nvec, d.ldc1, d.ldm1);
     tot[0] = g.xprep; tot[1] = g.k1; tot[2] = g.red; tot[3] = g.k2; tot[4] = g.wprep; tot[5] = g.k3;
 }
 
@@ -401,20 +524,20 @@ int operator_lp2(ngw_context* ctx, const OperatorArgs& a)
     if (B <= 0) return NGW_OK;
     NGW_TRY(lp2_prepare(ctx));
     const int kind = ctx->precision, nmu = ctx->n_intp, nv = ctx->nval;
-    const int S = std::max(k1_slots(nv), (nv - 1) / k1_bn_of_prec(kind) + 2);
+    const int S = lp2_slots(nv);
     NGW_TRY(ctx->tpart.ensure(sizeof(double) * (size_t)S * B * nmu));
-    NGW_TRY(ctx->ubuf.ensure(sizeof(double) * (size_t)B * nmu));
     NGW_TRY(ctx->lpwork.ensure(lp2_work_bytes(kind, nmu, nv, ctx->ncond, B)));
     LpOp d;
-    lp2_fill(ctx, kind, B, a.ldx, a.tda, ctx->lpwork.as<char>(), ctx->tpart.as<double>(), ctx->ubuf.as<double>(), d);
+    lp2_fill(ctx, kind, B, a.ldx, a.tda, ctx->lpwork.as<char>(), ctx->tpart.as<double>(), d);
     if (a.ldo != a.ldx || (a.v0 && a.ldv0 != a.ldx) || (a.y && a.ldy != a.ldx) || (a.acc && a.acc != a.out))
         return fail(NGW_ERR_ARG, "low-precision operator needs one leading dimension and acc == out");
     d.q = const_cast<double*>(a.v0);
     LpStage st{a.x, a.out, a.y, a.alpha, a.bx, a.by, a.v0 ? a.gamma : 0.0, a.acc ? 1 : 0};
-    int tot[6];
-    lp2_grid_counts(kind, d, tot);
-    const int* pre[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    return lp2_launch_any(kind, ctx->stream, nullptr, d, nullptr, st, 1, 0, pre, tot, 1, ctx);
+    const int c1 = ctx->lp_cfg[kind][0], c3 = ctx->lp_cfg[kind][1];
+    int tot[LP_NKERNEL];
+    lp2_grid_counts(kind, c1, c3, d, tot);
+    const int* pre[LP_NKERNEL] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    return lp2_launch_any(kind, ctx->stream, c1, c3, nullptr, d, nullptr, st, 1, 0, pre, tot, 1, ctx);
 }
 
 }  // namespace ngw

@@ -29,10 +29,25 @@ int fail(int code, const std::string& msg);
         if (s_ != NGW_OK) return s_;  \
     } while (0)
 
-// Device buffer that grows on demand and is freed with its owner.
+// Device buffer that grows on demand and is freed with its owner (movable,
+// not copyable).
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            ptr = o.ptr; bytes = o.bytes;
+            o.ptr = nullptr; o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int ensure(size_t need);
     void release();
     template <class T>
@@ -67,12 +82,14 @@ struct ngw_context {
     ngw::DevBuf diag_full;  // ncv: sqrtR^3   (applied to sqrtR*x)
     ngw::DevBuf diag_tda;   // ncv: sqrtR^2   (applied to x)
     // low-precision operator copies (FP32 / BF16 with per-band scales)
-    ngw::DevBuf psi32, m32, psibf, mbf, psi_scale, m_scale;
+    ngw::DevBuf psi_scale;
     bool lowp_ready = false;
     // pre-converted operands of the streamlined low-precision path
     bool lp2_ready = false;
     int ldc1 = 0, ldm1 = 0;
     ngw::DevBuf pct32, pc32, pctb, pcb, m32t, psiv32, lpwork;
+    // chosen {K1, K3} tile configuration per precision mode (ngw_autotune)
+    int lp_cfg[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     // workspaces
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;
     // host staging for drop-in calls

@@ -102,7 +102,6 @@ int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int n
 // Dispatch on ctx->precision.
 int operator_apply(ngw_context* ctx, const OperatorArgs& a);
 int lowp_prepare(ngw_context* ctx);
-int operator_lowp(ngw_context* ctx, const OperatorArgs& a);
 // Low-precision request descriptor (the operator of one system plus one
 // block of vectors) for the pre-converted-operand kernels.
 struct LpOp {
@@ -115,7 +114,7 @@ struct LpOp {
     int nmu, nv, nc, nvec, ldc1, ldm1;
     long ld; int tda;
     void* xlp_hi; void* xlp_lo; void* wlp_hi; void* wlp_lo;
-    float* t32; double* tpart; double* u;
+    float* t32; float* u32; double* tpart;
     double* q; double* w; double* a; double* b; int niter; int* kstep;
 };
 
@@ -128,12 +127,16 @@ struct LpStage {
 };
 
 int lp2_prepare(ngw_context* ctx);
-int lp2_launch_any(int kind, hipStream_t st, const LpOp* ops, const LpOp& op1, const LpStage* stg, const LpStage& st1,
-                   int nstage, int s, const int* const* pre, const int* tot, int nreq, ngw_context* prof);
+int lp2_launch_any(int kind, hipStream_t st, int c1, int c3, const LpOp* ops, const LpOp& op1, const LpStage* stg,
+                   const LpStage& st1, int nstage, int s, const int* const* pre, const int* tot, int nreq,
+                   ngw_context* prof);
+int lp2_slots(int nv);
 size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec);
-void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char* work, double* tpart, double* u,
-              LpOp& d);
-void lp2_grid_counts(int kind, const LpOp& d, int* tot);
+void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char* work, double* tpart, LpOp& d);
+// Kernels per low-precision operator application (xprep, K1, slot reduction,
+// K2, wprep, K3): the number of workgroup prefix tables a grouped launch needs.
+constexpr int LP_NKERNEL = 6;
+void lp2_grid_counts(int kind, int c1, int c3, const LpOp& d, int* tot);
 int operator_lp2(ngw_context* ctx, const OperatorArgs& a);
 
 // Lanczos pieces (lanczos.hip).

@@ -1,6 +1,6 @@
-// Operand loaders of the ISDF operator GEMMs (FP64 core and low-precision
-// core).  row(mn) returns a per-row handle; fetch() issues the raw loads of
-// element (mn, k) from clamped addresses; combine() forms the element.
+// Operand loaders of the FP64 ISDF operator GEMMs.  row(mn) returns a per-row
+// handle; fetch() issues the raw loads of element (mn, k) from clamped
+// addresses; combine() forms the element.
 #pragma once
 
 #include "gemm_core_lowp.h"
@@ -82,76 +82,6 @@ struct PairB {  // c_mn(nu) = Psi(nu, m_b) Psi(nu, n_b)  (P-vector build)
         r[1] = h.pn[nu];
     }
     __device__ double combine(const double (&r)[2]) const { return r[0] * r[1]; }
-};
-
-// ---------------------------------------------------------------- low-precision loaders (FP32 values)
-struct LpColA {  // A(m, k) = a[m + k*ld], FP32 source
-    static constexpr bool kContig = false;
-    static constexpr int NRAW = 1;
-    typedef float Raw;
-    typedef const float* Handle;
-    const float* a; long ld; int MN;
-    __device__ Handle row(int m) const { return a + m; }
-    __device__ void fetch(Handle h, int k, float (&r)[1]) const { r[0] = h[(long)k * ld]; }
-    __device__ float combine(const float (&r)[1]) const { return r[0]; }
-};
-struct LpColAT {  // A(m, k) = a[k + m*ld], FP32 source
-    static constexpr bool kContig = true;
-    static constexpr int NRAW = 1;
-    typedef float Raw;
-    typedef const float* Handle;
-    const float* a; long ld; int MN;
-    __device__ Handle row(int m) const { return a + (long)m * ld; }
-    __device__ void fetch(Handle h, int k, float (&r)[1]) const { r[0] = h[k]; }
-    __device__ float combine(const float (&r)[1]) const { return r[0]; }
-};
-struct LpColBd {  // B(k, n) = b[k + n*ld], FP64 source
-    static constexpr bool kContig = true;
-    static constexpr int NRAW = 1;
-    typedef double Raw;
-    typedef const double* Handle;
-    const double* b; long ld; int MN;
-    __device__ Handle row(int n) const { return b + (long)n * ld; }
-    __device__ void fetch(Handle h, int k, double (&r)[1]) const { r[0] = h[k]; }
-    __device__ float combine(const double (&r)[1]) const { return (float)r[0]; }
-};
-struct LpXprime {  // x'(c, j = v + nv b) = sigma_c s(v,c) x_b(v,c)
-    static constexpr bool kContig = false;
-    static constexpr int NRAW = 3;
-    typedef double Raw;
-    struct Handle { const double* x; const double* s; };
-    const double* x; const double* s; const float* sigma; long ldx; int nv; int MN;
-    __device__ Handle row(int j) const
-    {
-        int b = j / nv, v = j - b * nv;
-        return {x + (long)b * ldx + v, s ? s + v : nullptr};
-    }
-    __device__ void fetch(const Handle& h, int c, double (&r)[3]) const
-    {
-        long o = (long)c * nv;
-        r[0] = h.x[o];
-        r[1] = h.s ? h.s[o] : 1.0;
-        r[2] = sigma[c];
-    }
-    __device__ float combine(const double (&r)[3]) const { return (float)((r[1] * r[0]) * r[2]); }
-};
-struct LpWeighted {  // W(mu, j = v + nv b) = PsiV(mu, v) u(mu, b), FP64 sources
-    static constexpr bool kContig = true;
-    static constexpr int NRAW = 2;
-    typedef double Raw;
-    struct Handle { const double* pv; const double* ub; };
-    const double* psiv; long ldp; const double* u; long ldu; int nv; int MN;
-    __device__ Handle row(int j) const
-    {
-        int b = j / nv, v = j - b * nv;
-        return {psiv + (long)v * ldp, u + (long)b * ldu};
-    }
-    __device__ void fetch(const Handle& h, int mu, double (&r)[2]) const
-    {
-        r[0] = h.pv[mu];
-        r[1] = h.ub[mu];
-    }
-    __device__ float combine(const double (&r)[2]) const { return (float)(r[0] * r[1]); }
 };
 
 }  // namespace ngw

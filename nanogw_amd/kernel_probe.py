@@ -28,6 +28,8 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--tda", action="store_true")
+    ap.add_argument("--tiles", default="auto",
+                    help="K1,K3 tile configuration indices (e.g. 0,0), or 'auto' to run the autotuner first")
     a = ap.parse_args(argv)
 
     import numpy as np
@@ -44,6 +46,13 @@ def main(argv=None):
     ctx = IsdfContext(0)
     ctx.set_isdf(np.asfortranarray(psi), np.asfortranarray(mm), e1, a.nv)
     ctx.set_precision(a.precision)
+    if a.precision != "f64":
+        if a.tiles == "auto":
+            k1, k3 = ctx.autotune(a.nvec, reps=2)
+        else:
+            k1, k3 = (int(v) for v in a.tiles.split(","))
+            ctx.set_tile_config(k1, k3)
+        print(f"[probe] tile configurations K1 {k1}, K3 {k3}", flush=True)
     ncv = a.nv * a.nc
     dev = torch.device("cuda:0")
     x = torch.randn((a.nvec, ncv), dtype=torch.float64, device=dev)
@@ -51,13 +60,23 @@ def main(argv=None):
     torch.cuda.synchronize()
     ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, a.nvec, a.tda)  # first call prepares the operands
     ctx.synchronize()
+    ctx.profile_enable(True)
+    ctx.profile_reset()
     t = time.perf_counter()
     for _ in range(a.reps):
         ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, a.nvec, a.tda)
     ctx.synchronize()
     dt = (time.perf_counter() - t) / a.reps
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    fl = {"K1": 2.0 * a.nmu * a.nc * a.nv * a.nvec, "K2": 2.0 * a.nmu * a.nmu * a.nvec,
+          "K3": 2.0 * a.nc * a.nmu * a.nv * a.nvec}
+    per = "  ".join(f"{k} {prof[k][0] / max(prof[k][1], 1):.3f} ms ({fl[k] / (prof[k][0] / max(prof[k][1], 1)) / 1e9:.1f} "
+                    f"TFLOP/s)" for k in ("K1", "K2", "K3") if prof[k][1])
+    conv = prof["convert"][0] / a.reps
     print(f"[probe] operator {a.precision} Nmu={a.nmu} nv={a.nv} nc={a.nc} nvec={a.nvec}: "
-          f"{a.reps + 1} applications, {dt * 1e3:.3f} ms each after the first", flush=True)
+          f"{a.reps + 1} applications, {dt * 1e3:.3f} ms each after the first; {per}; convert {conv:.3f} ms",
+          flush=True)
     n = a.copy_bytes // 8
     src = torch.ones(n, dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)

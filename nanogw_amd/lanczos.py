@@ -202,6 +202,16 @@ class IsdfContext:
         _lib.check(self._lib.ngw_gemm_f64_dev(self._h, m, n, k, d_a, lda, d_b, ldb, d_c, ldc),
                    "ngw_gemm_f64_dev")
 
+    def autotune(self, nvec, reps=2):
+        """Time the K1/K3 tile configurations of the current precision on nvec
+        vectors and keep the fastest; returns (k1_index, k3_index)."""
+        a, b = C.c_int(), C.c_int()
+        _lib.check(self._lib.ngw_autotune(self._h, int(nvec), int(reps), C.byref(a), C.byref(b)), "ngw_autotune")
+        return a.value, b.value
+
+    def set_tile_config(self, k1, k3):
+        _lib.check(self._lib.ngw_set_tile_config(self._h, int(k1), int(k3)), "ngw_set_tile_config")
+
     # ------------------------------------------------------------ profiling
     KERNEL_CLASSES = ("K1", "K2", "K3", "lanczos_ab", "cfrac", "pvectors", "sigma", "convert")
 

@@ -64,6 +64,45 @@ def test_matvec_lowp_vs_oracle(ngw_ctx, wellcond, prec, tol, nvec):
         ngw_ctx.set_precision("f64")
 
 
+@pytest.fixture(scope="module")
+def tall():
+    """nc and Nmu above 256: several M tiles of every K1/K3 configuration,
+    the last one ragged."""
+    from nanogw_amd import synth
+
+    return synth.make_molecule(6.0, 0.5, 6, 300, 300, seed=2718, ridge=1e-6, cvt_iter=40)
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16x3", 1e-4), ("bf16", 3e-2)])
+def test_tile_configs_and_autotune_vs_oracle(ngw_ctx, tall, prec, tol):
+    """Every K1/K3 tile configuration (shape, slabs in flight) gives the
+    oracle's operator to the precision's tolerance, per column; the autotuner
+    returns valid indices and its choice is also correct."""
+    from oracle import oracle as O
+
+    m = tall
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    ngw_ctx.set_precision(prec)
+    try:
+        x = np.random.default_rng(4).normal(size=(m.nv * m.nc, 37))
+        ref = O.matvec_block(m.psi_intp, m.mmtrx, O.sqrt_r(m.e1, m.nv, m.nc), x, m.nv, m.nc)
+        outs = []
+        for cfg in range(4):
+            ngw_ctx.set_tile_config(cfg, cfg)
+            y = ngw_ctx.matvec_isdf_BLOCK(x)
+            err = max(rel_err(y[:, j], ref[:, j]) for j in range(x.shape[1]))
+            print("tile config", prec, cfg, err)
+            assert err < tol, (cfg, err)
+            outs.append(y)
+        k1, k3 = ngw_ctx.autotune(37, reps=1)
+        assert 0 <= k1 < 4 and 0 <= k3 < 4
+        y = ngw_ctx.matvec_isdf_BLOCK(x)
+        assert max(rel_err(y[:, j], ref[:, j]) for j in range(x.shape[1])) < tol
+    finally:
+        ngw_ctx.set_tile_config(0, 0)
+        ngw_ctx.set_precision("f64")
+
+
 @pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16x3", 1e-4)])
 def test_clenshaw_polynomial_vs_oracle(ngw_ctx, wellcond, prec, tol):
     """Low-precision p(H) (Chebyshev/Clenshaw form of the pinned monomial
