@@ -43,21 +43,52 @@ __device__ __forceinline__ int find_pre(const int* pre, int nreq, int bid)
 
 constexpr int L2K2 = 64;
 
-// Tile configurations of the K1 and K3 GEMMs: {BM, BN, D} with D the number
-// of slabs in flight (register prefetch depth).  Index 0 is the default; the
-// autotuner (ngw_autotune) times all of them on the caller's shape.
+// Tile configurations of the K1 and K3 GEMMs: {BM, BN, D, PM} with D the
+// number of slabs in flight (register prefetch depth) and PM the number of
+// M-groups the 8 XCDs are split into.  XCD x owns the tile rectangle
+// (M-group x mod PM) x (N-group x / PM): with PM > 1 each XCD touches only its
+// share of the A operand, which then stays in that XCD's L2, while the tiles
+// of one N column (M fastest inside the rectangle) share their B slab.  Index
+// 0 is the default; the autotuner (ngw_autotune) times all of them on the
+// caller's shape.
 struct LpCfg {
-    int bm, bn, d;
+    int bm, bn, d, pm;
 };
 constexpr int LP_NCFG = 4;
 constexpr LpCfg K1_CFG[2][LP_NCFG] = {
-    {{128, 128, 2}, {128, 128, 3}, {256, 128, 2}, {128, 64, 2}},    // FP32, BF16
-    {{128, 128, 2}, {128, 64, 2}, {128, 128, 1}, {128, 64, 3}}};    // BF16x3
+    {{128, 128, 2, 8}, {128, 128, 2, 4}, {128, 128, 2, 1}, {256, 128, 2, 8}},   // FP32, BF16
+    {{128, 128, 2, 8}, {128, 128, 2, 1}, {128, 64, 2, 8}, {128, 128, 1, 8}}};   // BF16x3
 constexpr LpCfg K3_CFG[2][LP_NCFG] = {
-    {{128, 128, 2}, {128, 128, 3}, {64, 128, 2}, {64, 128, 3}},
-    {{128, 128, 2}, {64, 128, 2}, {128, 128, 1}, {64, 128, 3}}};
+    {{128, 128, 2, 4}, {128, 128, 2, 2}, {128, 128, 2, 1}, {64, 128, 2, 8}},
+    {{128, 128, 2, 4}, {128, 128, 2, 1}, {64, 128, 2, 8}, {128, 128, 1, 4}}};
+constexpr int NXCD = 8;
 __host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : 0; }
 static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : 0; }
+
+// Workgroups of one request's GEMM: NXCD rectangles of the largest size
+// (workgroups of a smaller rectangle beyond its tiles exit at once).
+__host__ __device__ inline int xcd_grid(int mt, int nt, int pm)
+{
+    const int pn = NXCD / pm;
+    return NXCD * ((mt + pm - 1) / pm) * ((nt + pn - 1) / pn);
+}
+
+// Tile of a request-local workgroup index (the request's workgroups start at
+// a multiple of NXCD, so local % NXCD is the XCD of round-robin dispatch);
+// false for a workgroup past its rectangle.
+__device__ __forceinline__ bool xcd_rect_tile(int local, int mt, int nt, int pm, int& tm, int& tn)
+{
+    const int pn = NXCD / pm;
+    const int x = local % NXCD, i = local / NXCD;
+    const int mg = x % pm, ng = x / pm;
+    const int m_lo = mg * mt / pm, m_hi = (mg + 1) * mt / pm;
+    const int n_lo = ng * nt / pn, n_hi = (ng + 1) * nt / pn;
+    const int rows = m_hi - m_lo;
+    if (rows <= 0 || i >= rows * (n_hi - n_lo)) return false;
+    tm = m_lo + i % rows;
+    tn = n_lo + i / rows;
+    return true;
+}
 
 // Narrowest K1 tile of any configuration: sizes the slot partial sums.
 constexpr int K1_BN_MIN = 64;
@@ -226,7 +257,7 @@ __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, co
 }
 
 // ---------------------------------------------------------------- GEMMs
-template <int KIND, int BM, int BN, int D>
+template <int KIND, int BM, int BN, int D, int PM>
 __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
@@ -235,10 +266,10 @@ __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, co
         SmemV2<KIND, BM, BN> ab;
         float red[HALF][BN + 1];
     } sm;
-    LP_PICK_TILE();
+    LP_PICK_OP();
     const int nv = op.nv, N = nv * op.nvec;
-    const int mt = (op.nmu + BM - 1) / BM;
-    const int tm = local % mt, tn = local / mt;
+    int tm, tn;
+    if (!xcd_rect_tile(local, (op.nmu + BM - 1) / BM, (N + BN - 1) / BN, PM, tm, tn)) return;
     const int m0 = tm * BM, n0 = tn * BN;
     AccLp<BM, BN> acc;
     gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pct_hi, (const E*)op.pct_lo, op.ldc1, op.nmu},
@@ -330,17 +361,18 @@ __global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, c
     }
 }
 
-template <int KIND, int BM, int BN, int D>
+template <int KIND, int BM, int BN, int D, int PM>
 __global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                    int nstage, int s, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
     __shared__ SmemV2<KIND, BM, BN> sm;
-    LP_PICK_TILE();
+    LP_PICK_OP();
     const LpStage sg = ops ? st[r * nstage + s] : st1;
     const int nv = op.nv, N = nv * op.nvec;
-    const int mt = (op.nc + BM - 1) / BM;
-    const int m0 = (local % mt) * BM, n0 = (local / mt) * BN;
+    int tm, tn;
+    if (!xcd_rect_tile(local, (op.nc + BM - 1) / BM, (N + BN - 1) / BN, PM, tm, tn)) return;
+    const int m0 = tm * BM, n0 = tn * BN;
     AccLp<BM, BN> acc;
     gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc},
                                   OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0, n0,
@@ -378,11 +410,11 @@ static LpGrid lp_grid(int prec, int c1, int c3, int nmu, int nv, int nc, int nve
     const LpCfg k1 = K1_CFG[cfg_set_prec(prec)][c1], k3 = K3_CFG[cfg_set_prec(prec)][c3];
     LpGrid g;
     g.xprep = nvec * ((nv + 31) / 32) * (ldc1 / 64);
-    g.k1 = ((nmu + k1.bm - 1) / k1.bm) * ((nv * nvec + k1.bn - 1) / k1.bn);
+    g.k1 = xcd_grid((nmu + k1.bm - 1) / k1.bm, (nv * nvec + k1.bn - 1) / k1.bn, k1.pm);
     g.red = (int)(((long)(nvec + 1) * ldm1 + 255) / 256);
     g.k2 = ((nmu + L2K2 - 1) / L2K2) * ((nvec + L2K2 - 1) / L2K2);
     g.wprep = (int)(((long)(nv * nvec + 1) * (ldm1 / 8) + 255) / 256);
-    g.k3 = ((nc + k3.bm - 1) / k3.bm) * ((nv * nvec + k3.bn - 1) / k3.bn);
+    g.k3 = xcd_grid((nc + k3.bm - 1) / k3.bm, (nv * nvec + k3.bn - 1) / k3.bn, k3.pm);
     return g;
 }
 
@@ -390,7 +422,7 @@ template <int KIND, int C>
 static void launch_k1(hipStream_t st, int nblk, const LpOp* ops, const LpOp& op1, const int* pre, int nreq)
 {
     constexpr LpCfg c = K1_CFG[cfg_set(KIND)][C];
-    hipLaunchKernelGGL((k1v2_kernel<KIND, c.bm, c.bn, c.d>), dim3(nblk), dim3(GNT), 0, st, ops, op1, pre, nreq);
+    hipLaunchKernelGGL((k1v2_kernel<KIND, c.bm, c.bn, c.d, c.pm>), dim3(nblk), dim3(GNT), 0, st, ops, op1, pre, nreq);
 }
 
 template <int KIND, int C>
@@ -398,7 +430,7 @@ static void launch_k3(hipStream_t st, int nblk, const LpOp* ops, const LpOp& op1
                       const LpStage& st1, int nstage, int s, const int* pre, int nreq)
 {
     constexpr LpCfg c = K3_CFG[cfg_set(KIND)][C];
-    hipLaunchKernelGGL((k3v2_kernel<KIND, c.bm, c.bn, c.d>), dim3(nblk), dim3(GNT), 0, st, ops, op1, stg, st1,
+    hipLaunchKernelGGL((k3v2_kernel<KIND, c.bm, c.bn, c.d, c.pm>), dim3(nblk), dim3(GNT), 0, st, ops, op1, stg, st1,
                        nstage, s, pre, nreq);
 }
 
