@@ -58,9 +58,11 @@ def build_workload(cfg, ctx, seed_index=0):
 
     R, h, nv, nc, nmu, niter, nqp = synth.CONFIGS[cfg]
     t = time.time()
+    ridge = 0.0 if cfg == "c1" else synth.BENCH_RIDGE
     mol = synth.make_molecule(R, h, nv, nc, nmu, synth.config_seed(int(cfg[1:]), seed_index),
-                              cvt_iter=300 if nmu <= 500 else 60, kernel_ctx=ctx,
+                              cvt_iter=300 if nmu <= 500 else 60, kernel_ctx=ctx, ridge=ridge,
                               device=torch.device("cuda:0"), keep_zeta=False)
+    mol.ridge = ridge
     log(f"[bench] {cfg}: ngrid={mol.ngrid} nfft={mol.nfft} nv={nv} nc={nc} nmu={nmu} "
         f"built in {time.time() - t:.1f}s")
     # QP states nearest the gap first: HOMO, LUMO, HOMO-1, LUMO+1, ...
@@ -160,7 +162,7 @@ def run_batched(args):
     for i, sp in enumerate(specs):
         mols.append(synth.make_molecule(sp["radius"], sp["h"], sp["nv"], sp["nc"], sp["n_intp"], sp["seed"],
                                         cvt_iter=20, kernel_ctx=ctx, device=torch.device("cuda:0"),
-                                        keep_zeta=False))
+                                        keep_zeta=False, ridge=synth.BENCH_RIDGE))
         if i % 8 == 7 or i == len(specs) - 1:
             log(f"[bench] c4: {i + 1}/{len(specs)} molecules built ({time.time() - t:.1f}s)")
     niter = specs[0]["niter"]
@@ -236,7 +238,7 @@ def run_stream(args):
         sp = specs[i]
         mols[i] = synth.make_molecule(sp["radius"], sp["h"], sp["nv"], sp["nc"], sp["n_intp"], sp["seed"],
                                       cvt_iter=20 if sp["n_intp"] < 2000 else 60, kernel_ctx=ctx,
-                                      device=torch.device("cuda:0"), keep_zeta=False)
+                                      device=torch.device("cuda:0"), keep_zeta=False, ridge=synth.BENCH_RIDGE)
         if k % 8 == 7 or k == len(used) - 1:
             log(f"[bench] c5: {k + 1}/{len(used)} molecules built ({time.time() - t:.1f}s)")
     order = list(mols)
@@ -486,7 +488,8 @@ def main():
         "dtype": args.precision,
         "data": "synthetic (seeded Gaussian-Hermite molecule, CVT ISDF points, device V build)",
         "config": {"workload": f"{args.config}: R={mol.radius} h={mol.h} ngrid={mol.ngrid} nv={nv} nc={nc} "
-                               f"Nmu={nmu} N_b={nb} niter={niter} npoly={npoly} nen={nen}; {args.mode} mode, "
+                               f"Nmu={nmu} N_b={nb} niter={niter} npoly={npoly} nen={nen}, zeta ridge "
+                               f"{mol.ridge:g}; {args.mode} mode, "
                                f"one QP state per step" + (", hipGraph replay" if args.graph else ""),
                    "qp_states": states[args.warmup:], "parallelism": "single GPU",
                    "tile_configs": tiles},

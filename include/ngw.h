@@ -143,6 +143,16 @@ int ngw_sigma_lanczos_bands(ngw_context* ctx, int nqp, const int* qp_band, int b
                             int tamm_d, int max_cols, double* sigma_c);
 
 /*
+ * Bare exchange of QP states in the ISDF representation, replaces the
+ * diagonal matrix elements of Zfock_exchange (SIGMA/fock_exchange.F90z:12-229)
+ * with the operator's own Coulomb kernel:
+ *   sigx(q) = - sum_{v < nval} sum_{mu,nu} c(mu) M(mu,nu) c(nu),
+ *   c(mu) = Psi(mu, qp_band(q)) Psi(mu, v)          (Ry)
+ * qp_band(nqp) 0-based bands; sigx(nqp) output.
+ */
+int ngw_sigma_x_isdf(ngw_context* ctx, int nqp, const int* qp_band, double* sigx);
+
+/*
  * QP energies by Newton-Raphson on the cubic spline of
  * f(E) = E_dft + Sigma_x + Sigma_c(E) - V_xc - E (eV), replaces the diagonal
  * block of printsigma (SIGMA/printsigma.F90:97-128) with newtonr

@@ -37,6 +37,7 @@ _SIGS = {
                            C.c_int, C.c_int, _dp], C.c_int),
     "ngw_sigma_lanczos_bands": ([_vp, C.c_int, _ip, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                  _dp, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
+    "ngw_sigma_x_isdf": ([_vp, C.c_int, _ip, _dp], C.c_int),
     "ngw_qp_newton": ([_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                        _ip], C.c_int),
     "ngw_polyfit_sqrt": ([C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp], C.c_int),

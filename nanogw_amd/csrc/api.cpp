@@ -567,6 +567,23 @@ int ngw_sigma_lanczos_bands(ngw_context* ctx, int nqp, const int* qp_band, int b
     return NGW_OK;
 }
 
+int ngw_sigma_x_isdf(ngw_context* ctx, int nqp, const int* qp_band, double* sigx)
+{
+    NGW_TRY(check_ctx(ctx));
+    if (nqp <= 0 || !qp_band || !sigx) return fail(NGW_ERR_ARG, "need nqp > 0 and arrays");
+    for (int i = 0; i < nqp; ++i)
+        if (qp_band[i] < 0 || qp_band[i] >= ctx->nband) return fail(NGW_ERR_ARG, "qp_band out of range");
+    hipStream_t s = ctx->stream;
+    NGW_TRY(ctx->staging.ensure(sizeof(double) * nqp + sizeof(int) * nqp + 256));
+    double* dsig = ctx->staging.as<double>();
+    int* dqp = reinterpret_cast<int*>(dsig + nqp);
+    NGW_HIP(hipMemcpyAsync(dqp, qp_band, sizeof(int) * nqp, hipMemcpyHostToDevice, s));
+    NGW_TRY(sigma_x_isdf(ctx, nqp, dqp, dsig));
+    NGW_HIP(hipMemcpyAsync(sigx, dsig, sizeof(double) * nqp, hipMemcpyDeviceToHost, s));
+    NGW_HIP(hipStreamSynchronize(s));
+    return NGW_OK;
+}
+
 int ngw_qp_newton(ngw_context* ctx, int nqp, int nen, double deltae_ev, const double* edft,
                   const double* en0, const double* sigx, const double* vxc, const double* sigma_c,
                   double* eqp, double* zfac, double* sigma0, double* imsig, int* ierr)

@@ -136,6 +136,82 @@ struct SlabV2 {
     }
 };
 
+// B-operand slab of W(j, k) = p(v, k) u(b, k), j = v + nv b, formed while
+// loading: p (nv rows) and u (rows b; row `zero_row` all zeros) are FP32 row
+// arrays of leading dimension ld, small enough to stay in L2, so W never
+// exists in memory.  Rows j >= rows read u's zero row.  Each thread handles
+// CHK consecutive k of one row; the product and the conversion happen at
+// LDS-store time.
+template <int KIND, int MN>
+struct SlabW {
+    typedef V2Traits<KIND> Tr;
+    static constexpr int CHK = MN * Tr::BK / GNT;   // elements per thread per slab
+    static constexpr int TPR = Tr::BK / CHK;        // threads per row
+    static_assert(CHK % 8 == 0 && TPR * CHK == Tr::BK && MN * TPR == GNT, "slab split");
+    struct Raw {
+        f32x4 pv[CHK / 4], pu[CHK / 4];
+    };
+    const float* prow;
+    const float* urow;
+    int row, kq;
+
+    __device__ __forceinline__ void init(const float* p, const float* u, long ld, int nv, int rows, int zero_row,
+                                         int mn0)
+    {
+        row = threadIdx.x / TPR;
+        kq = (threadIdx.x % TPR) * CHK;
+        const int j = mn0 + row;
+        const int b = j < rows ? j / nv : zero_row, v = j < rows ? j - (j / nv) * nv : 0;
+        prow = p + (long)v * ld + kq;
+        urow = u + (long)b * ld + kq;
+    }
+    __device__ __forceinline__ void fetch(int k0, Raw& raw) const
+    {
+#pragma unroll
+        for (int i = 0; i < CHK / 4; ++i) {
+            raw.pv[i] = gload<f32x4>(prow + k0 + 4 * i);
+            raw.pu[i] = gload<f32x4>(urow + k0 + 4 * i);
+        }
+    }
+    __device__ __forceinline__ void store(const Raw& raw, typename Tr::elem (*dst)[MN][Tr::BK + Tr::PADV]) const
+    {
+        float w[CHK];
+#pragma unroll
+        for (int i = 0; i < CHK / 4; ++i) {
+            const f32x4 q = raw.pv[i] * raw.pu[i];
+            w[4 * i] = q.x;
+            w[4 * i + 1] = q.y;
+            w[4 * i + 2] = q.z;
+            w[4 * i + 3] = q.w;
+        }
+        if constexpr (KIND == LP_F32) {
+#pragma unroll
+            for (int i = 0; i < CHK / 4; ++i)
+                *reinterpret_cast<f32x4*>(&dst[0][row][kq + 4 * i]) =
+                    f32x4{w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]};
+        } else {
+#pragma unroll
+            for (int i = 0; i < CHK / 8; ++i) {
+                unsigned hi[4], lo[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float a = w[8 * i + 2 * e], c = w[8 * i + 2 * e + 1];
+                    const unsigned short ha = f32_to_bf16_bits(a), hc = f32_to_bf16_bits(c);
+                    hi[e] = (unsigned)ha | ((unsigned)hc << 16);
+                    if constexpr (Tr::NSPLIT == 2) {
+                        const unsigned short la = f32_to_bf16_bits(a - bf16_bits_to_f32(ha));
+                        const unsigned short lc = f32_to_bf16_bits(c - bf16_bits_to_f32(hc));
+                        lo[e] = (unsigned)la | ((unsigned)lc << 16);
+                    }
+                }
+                *reinterpret_cast<u32x4*>(&dst[0][row][kq + 8 * i]) = u32x4{hi[0], hi[1], hi[2], hi[3]};
+                if constexpr (Tr::NSPLIT == 2)
+                    *reinterpret_cast<u32x4*>(&dst[1][row][kq + 8 * i]) = u32x4{lo[0], lo[1], lo[2], lo[3]};
+            }
+        }
+    }
+};
+
 // MFMA work on one LDS slab: the wave's (BM/2) x (BN/2) block of the tile.
 template <int KIND, int BM, int BN>
 __device__ __forceinline__ void mfma_slab(const SmemV2<KIND, BM, BN>& sm, int cur, AccLp<BM, BN>& acc)

@@ -53,14 +53,15 @@ constexpr int L2K2 = 64;
 // caller's shape.
 struct LpCfg {
     int bm, bn, d, pm;
+    int fw = 0;  // K3 only: 1 = W formed in the slab loader (no wprep pass)
 };
 constexpr int LP_NCFG = 4;
 constexpr LpCfg K1_CFG[2][LP_NCFG] = {
     {{128, 128, 2, 8}, {128, 128, 2, 4}, {128, 128, 2, 1}, {256, 128, 2, 8}},   // FP32, BF16
     {{128, 128, 2, 8}, {128, 128, 2, 1}, {128, 64, 2, 8}, {128, 128, 1, 8}}};   // BF16x3
 constexpr LpCfg K3_CFG[2][LP_NCFG] = {
-    {{128, 128, 2, 4}, {128, 128, 2, 2}, {128, 128, 2, 1}, {64, 128, 2, 8}},
-    {{128, 128, 2, 4}, {128, 128, 2, 1}, {64, 128, 2, 8}, {128, 128, 1, 4}}};
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 1}, {128, 128, 2, 2}, {128, 128, 2, 8, 1}},
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 1}, {128, 128, 2, 1}, {64, 128, 2, 8}}};
 constexpr int NXCD = 8;
 __host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : 0; }
 static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : 0; }
@@ -239,7 +240,8 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
 }
 
 // t32(b, nu) = sum of the K1 slots covering vector b (fixed order); zero
-// padding and zero row b = nvec.
+// padding and zero row b = nvec (also written into u, whose rows b < nvec
+// come from K2).
 __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq, int bn)
 {
     LP_PICK_OP();
@@ -254,6 +256,7 @@ __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, co
         out = (float)acc;
     }
     op.t32[i] = out;
+    if (b == op.nvec) op.u32[i] = 0.0f;
 }
 
 // ---------------------------------------------------------------- GEMMs
@@ -361,7 +364,7 @@ __global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, c
     }
 }
 
-template <int KIND, int BM, int BN, int D, int PM>
+template <int KIND, int BM, int BN, int D, int PM, int FW>
 __global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                    int nstage, int s, const int* pre, int nreq)
 {
@@ -374,9 +377,17 @@ __global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, co
     if (!xcd_rect_tile(local, (op.nc + BM - 1) / BM, (N + BN - 1) / BN, PM, tm, tn)) return;
     const int m0 = tm * BM, n0 = tn * BN;
     AccLp<BM, BN> acc;
-    gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc},
-                                  OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0, n0,
-                                  op.ldm1, acc, sm);
+    if constexpr (FW) {
+        SlabV2<KIND, BM> la;
+        SlabW<KIND, BN> lb;
+        la.init(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc}, m0);
+        lb.init(op.psiv32, op.u32, op.ldm1, nv, N, op.nvec, n0);
+        gemm_v2_run<KIND, BM, BN, D>(la, lb, op.ldm1, acc, sm);
+    } else {
+        gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc},
+                                      OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0,
+                                      n0, op.ldm1, acc, sm);
+    }
     OperatorEpilogue ep;
     ep.x = sg.x; ep.ldx = op.ld;
     ep.d = op.tda ? op.dtda : op.dfull;
@@ -430,7 +441,7 @@ static void launch_k3(hipStream_t st, int nblk, const LpOp* ops, const LpOp& op1
                       const LpStage& st1, int nstage, int s, const int* pre, int nreq)
 {
     constexpr LpCfg c = K3_CFG[cfg_set(KIND)][C];
-    hipLaunchKernelGGL((k3v2_kernel<KIND, c.bm, c.bn, c.d, c.pm>), dim3(nblk), dim3(GNT), 0, st, ops, op1, stg, st1,
+    hipLaunchKernelGGL((k3v2_kernel<KIND, c.bm, c.bn, c.d, c.pm, c.fw>), dim3(nblk), dim3(GNT), 0, st, ops, op1, stg, st1,
                        nstage, s, pre, nreq);
 }
 
@@ -462,7 +473,7 @@ static int lp_launch(hipStream_t st, int c1, int c3, const LpOp* ops, const LpOp
         ProfScope ps(prof, KC_K2);
         hipLaunchKernelGGL(k2v2_kernel, dim3(tot[3]), dim3(GNT), 0, st, ops, op1, pre[3], nreq);
     }
-    {
+    if (!K3_CFG[cfg_set(KIND)][c3].fw) {
         ProfScope ps(prof, KC_CONVERT);
         hipLaunchKernelGGL(wprep_kernel<KIND>, dim3(tot[4]), dim3(256), 0, st, ops, op1, pre[4], nreq);
     }
@@ -506,7 +517,7 @@ size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec)
     const size_t es = kind == PREC_F32 ? 4 : 2, ns = kind == PREC_BF16X3 ? 2 : 1;
     const size_t ldc1 = padk(nc), ldm1 = padk(nmu), N = (size_t)nv * nvec;
     return ns * (align256(es * (N + 1) * ldc1) + align256(es * (N + 1) * ldm1)) +
-           align256(4 * (size_t)(nvec + 1) * ldm1) + align256(4 * (size_t)nvec * ldm1);
+           2 * align256(4 * (size_t)(nvec + 1) * ldm1);
 }
 
 // Fill the descriptor of one request from its operator and workspaces.

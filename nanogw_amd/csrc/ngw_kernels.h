@@ -155,5 +155,6 @@ int sigma_assemble(hipStream_t s, const double* d_spec, const double* d_zz, cons
                    int nband, int band0, int nval, int tda, double* d_sigma);
 int qp_newton(hipStream_t s, int nqp, int nen, double deltae_ev, const double* d_in,
               double* d_out, double* d_scratch);
+int sigma_x_isdf(ngw_context* ctx, int nqp, const int* d_qp, double* d_sigx);
 
 }  // namespace ngw

@@ -102,6 +102,14 @@ module ngw_c
       complex(c_double_complex), intent(out) :: sigma_c(*)
     end function ngw_sigma_lanczos_bands
 
+    integer(c_int) function ngw_sigma_x_isdf(ctx, nqp, qp_band, sigx) bind(C, name='ngw_sigma_x_isdf')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nqp
+      integer(c_int), intent(in) :: qp_band(*)
+      real(c_double), intent(out) :: sigx(*)
+    end function ngw_sigma_x_isdf
+
     integer(c_int) function ngw_qp_newton(ctx, nqp, nen, deltae_ev, edft, en0, sigx, vxc, &
                                           sigma_c, eqp, zfac, sigma0, imsig, ierr) &
       bind(C, name='ngw_qp_newton')

@@ -160,6 +160,15 @@ class IsdfContext:
                 _p(poly), len(poly) - 1, int(tamm_d), int(max_cols), outp), "ngw_sigma_lanczos_bands")
         return out
 
+    def sigma_x(self, qp):
+        """Bare exchange (Ry) of 0-based QP bands in the ISDF representation
+        (the diagonal of fock_exchange.F90z:12-229 with the operator's M)."""
+        qp = np.ascontiguousarray(qp, np.int32)
+        out = np.zeros(len(qp))
+        _lib.check(self._lib.ngw_sigma_x_isdf(self._h, len(qp), qp.ctypes.data_as(C.POINTER(C.c_int)), _p(out)),
+                   "ngw_sigma_x_isdf")
+        return out
+
     def qp_newton(self, sigma_c_ev, deltae_ev, edft_ev, en0_ev, sigx_ev, vxc_ev):
         """printsigma.F90:97-128 on the device; all energies eV; sigma_c (nen, nqp)."""
         sc = np.require(np.asarray(sigma_c_ev, np.complex128), np.complex128, ["F_CONTIGUOUS"])

@@ -223,3 +223,21 @@ def test_isdf_coulomb_kernel_vs_oracle(ngw_ctx, mini):
     m2 = ngw_ctx.isdf_coulomb_kernel(z[: mini.ngrid], mini.gidx, nf, mini.h, mini.trunc)
     r2 = O.isdf_kernel(mini.zeta, mini.gidx, nf, mini.h, mini.trunc)
     assert rel_err(m2, r2) < 1e-10
+
+
+def test_sigma_x_isdf(ngw_ctx, mini):
+    """Device bare exchange: the numpy expression on the same inputs, and the
+    closed form sum_v sum_mu (Psi_m Psi_v)^2 when M is the identity."""
+    from nanogw_amd import synth
+
+    m = mini
+    qp = [m.nv - 1, m.nv, m.nv + 2]
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    got = ngw_ctx.sigma_x(qp)
+    ref = synth.exchange_diag(m.psi_intp, m.mmtrx, m.nv, qp)
+    assert rel_err(got, ref) < 1e-12
+    eye = np.eye(m.n_intp)
+    ngw_ctx.set_isdf(m.psi_intp, eye, m.e1, m.nv)
+    got = ngw_ctx.sigma_x(qp)
+    closed = [-np.sum((m.psi_intp[:, [q]] * m.psi_intp[:, :m.nv]) ** 2) for q in qp]
+    assert rel_err(got, np.array(closed)) < 1e-13
