@@ -25,7 +25,9 @@ Beside the timed run the JSON line carries
                 fingerprint matches;
   roofline      the dominant GEMM's algorithmic FLOP / its summed HIP-event
                 time over the timed region, against the MI355X dense peak of
-                its arithmetic;
+                its arithmetic; traffic = its HBM bytes per launch from the
+                copy-calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
+                profiles/pmc_traffic_c3.md (default c3 served run only);
   cpu_baseline  the CPU oracle on a bounded sample, extrapolated.
 Only one GPU is supported.
 """
@@ -42,7 +44,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "bf16x3": 2.5e15}
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "bf16x3": 2.5e15, "f16": 2.5e15}
 RYD = 13.60569193
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
@@ -135,15 +137,35 @@ def fp64_sample(ctx, mol, poly, niter, steps, chunk=200):
                     f"{steps} Lanczos steps, extrapolated to niter and N_b"}
 
 
-def roofline_from_profile(prof, flops, precision):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c3.md")
+
+
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of a GEMM kernel at the served c3 launch shape
+    (3200 vectors), from the copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE
+    summary committed under profiles/ (lines 'traffic <kernel> <precision>
+    <bytes>'); None when not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 4 and p[0] == "traffic" and p[1] == kernel and p[2] == precision:
+                    return float(p[3])
+    except OSError:
+        pass
+    return None
+
+
+def roofline_from_profile(prof, flops, precision, traffic_shape_ok=False):
     dom = max(flops, key=lambda k: prof[k][0])
     ms, cnt = prof[dom]
     if cnt == 0 or ms <= 0:
         return None
     ach = flops[dom] / (ms / 1000.0)
     peak = PEAK[precision]
+    traffic = measured_traffic(dom, precision) if traffic_shape_ok else None
     return {"bound": "mfma", "kernel": dom, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": ach / peak, "traffic": None, "avg_launch_us": ms * 1000.0 / cnt, "launches": cnt,
+            "frac": ach / peak, "traffic": traffic, "avg_launch_us": ms * 1000.0 / cnt, "launches": cnt,
             "flop_per_launch": flops[dom] / cnt}
 
 
@@ -305,7 +327,7 @@ def main():
     ap.add_argument("--large-frac", type=float, default=0.1, help="c5: share of R = 13, nv = 100 requests")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch each step directly")
     ap.add_argument("--mode", default="served", choices=["served", "single"])
-    ap.add_argument("--precision", default="bf16", choices=["f64", "f32", "bf16", "bf16x3"])
+    ap.add_argument("--precision", default="f16", choices=["f64", "f32", "bf16", "bf16x3", "f16"])
     ap.add_argument("--graph", action="store_true",
                     help="served mode: replay each Lanczos step as a hipGraph (no per-kernel timers)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -439,7 +461,10 @@ def main():
     mv_total = float(cols_total) * niter * npoly
     flops = {"K1": 2.0 * nmu * nc * nv * mv_total, "K2": 2.0 * nmu * nmu * mv_total,
              "K3": 2.0 * nc * nmu * nv * mv_total}
-    roof = roofline_from_profile(prof, flops, args.precision) if prof["K1"][1] else None
+    # the committed PMC traffic was measured at the launch shape of the default
+    # served c3 run: both timed states (2 x 1600 vectors) in every launch
+    shape_ok = args.config == "c3" and args.mode == "served" and args.steps == 2
+    roof = roofline_from_profile(prof, flops, args.precision, shape_ok) if prof["K1"][1] else None
     f_mv = 4.0 * nmu * nv * nc + 2.0 * nmu * nmu
 
     # QP error against the stored FP64 reference of this molecule

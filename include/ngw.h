@@ -43,6 +43,8 @@ extern "C" {
                             M t in FP32 */
 #define NGW_PREC_BF16X3 3 /* as BF16 with each operand split into BF16 head + tail, three
                              MFMAs per product (about 16 mantissa bits) */
+#define NGW_PREC_F16 4   /* C x and C^T w in FP16 (11 mantissa bits) with exact power-of-two
+                            scales per band and per vector, FP32 accumulate; M t in FP32 */
 
 typedef struct ngw_context ngw_context;
 
@@ -242,7 +244,7 @@ int ngw_profile_read(ngw_context* ctx, int kernel_class, double* total_ms, long*
 
 /*
  * C = A B through the low-precision MFMA core (kind 0 FP32, 1 BF16,
- * 2 BF16x3), FP64 device arrays in and out; exported for the library's own
+ * 2 BF16x3, 3 FP16 unscaled), FP64 device arrays in and out; exported for the library's own
  * tests of the matrix-core lane layouts.
  */
 int ngw_gemm_lowp_dev(ngw_context* ctx, int kind, int m, int n, int k, const double* d_a, long lda,

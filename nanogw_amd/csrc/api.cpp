@@ -228,7 +228,8 @@ int ngw_destroy(ngw_context* ctx)
                       &ctx->psi_scale,
                       &ctx->tpart, &ctx->ubuf, &ctx->w0, &ctx->w1, &ctx->w2, &ctx->w3, &ctx->w4,
                       &ctx->ab, &ctx->zz, &ctx->spec, &ctx->misc, &ctx->misc2, &ctx->staging,
-                      &ctx->pct32, &ctx->pc32, &ctx->pctb, &ctx->pcb, &ctx->m32t, &ctx->psiv32, &ctx->lpwork};
+                      &ctx->pct32, &ctx->pc32, &ctx->pctb, &ctx->pcb, &ctx->pcth, &ctx->pch, &ctx->m32t,
+                      &ctx->psiv32, &ctx->psiv_scale, &ctx->lpwork};
     for (DevBuf* b : bufs) b->release();
     for (auto& r : ctx->prof_recs) {
         (void)hipEventDestroy(r.a);
@@ -304,7 +305,7 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
 int ngw_set_precision(ngw_context* ctx, int precision)
 {
     if (!ctx) return fail(NGW_ERR_ARG, "null context");
-    if (precision < NGW_PREC_F64 || precision > NGW_PREC_BF16X3) return fail(NGW_ERR_ARG, "bad precision");
+    if (precision < NGW_PREC_F64 || precision > NGW_PREC_F16) return fail(NGW_ERR_ARG, "bad precision");
     ctx->precision = precision;
     if (precision != NGW_PREC_F64 && ctx->n_intp > 0) {
         NGW_HIP(hipSetDevice(ctx->device));

@@ -15,8 +15,9 @@ namespace ngw {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
-enum LowpKind { LP_F32 = 0, LP_BF16 = 1, LP_BF16X3 = 2 };
+enum LowpKind { LP_F32 = 0, LP_BF16 = 1, LP_BF16X3 = 2, LP_F16 = 3 };
 
 __device__ __forceinline__ unsigned short f32_to_bf16_bits(float f)
 {
@@ -27,6 +28,21 @@ __device__ __forceinline__ unsigned short f32_to_bf16_bits(float f)
 __device__ __forceinline__ float bf16_bits_to_f32(unsigned short h)
 {
     return __uint_as_float(((unsigned int)h) << 16);
+}
+__device__ __forceinline__ unsigned short f32_to_f16_bits(float f)
+{
+    return __builtin_bit_cast(unsigned short, (_Float16)f);  // v_cvt_f16_f32, round to nearest even
+}
+
+// Smallest power of two >= m (1 for m == 0): exact scale factors for the FP16
+// operands, whose range ends at 65504.
+__device__ __forceinline__ float pow2_ceil(float m)
+{
+    const unsigned u = __float_as_uint(m) & 0x7FFFFFFFu;
+    if (u == 0u) return 1.0f;
+    unsigned e = u >> 23;
+    if (u & 0x7FFFFFu) ++e;
+    return __uint_as_float(e << 23);
 }
 
 template <int BM, int BN>

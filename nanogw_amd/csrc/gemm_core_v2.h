@@ -1,5 +1,5 @@
 // Streamlined low-precision MFMA GEMM core for gfx950 (FP32 16x16x4, BF16
-// 16x16x32, BF16x3 = head/tail split with three MFMAs per product).
+// and FP16 16x16x32, BF16x3 = head/tail split with three MFMAs per product).
 //
 // Both operands are pre-converted, k-contiguous row arrays in global memory
 // (row r, k at base[r * ld + k], ld a multiple of the slab depth, the padding
@@ -34,6 +34,19 @@ struct V2Traits<LP_BF16X3> {
     typedef unsigned short elem;
     static constexpr int BK = 32, PADV = 8, NSPLIT = 2;
 };
+template <>
+struct V2Traits<LP_F16> {
+    typedef unsigned short elem;
+    static constexpr int BK = 32, PADV = 8, NSPLIT = 1;
+};
+
+// FP32 value -> 16-bit element of a BF16 / BF16x3-head / FP16 operand.
+template <int KIND>
+__device__ __forceinline__ unsigned short to_e16(float v)
+{
+    if constexpr (KIND == LP_F16) return f32_to_f16_bits(v);
+    else return f32_to_bf16_bits(v);
+}
 
 // A pre-converted operand: rows x ld elements (hi part; lo part for BF16x3),
 // followed by one all-zero row (row `rows`), which tiles read for their rows
@@ -148,6 +161,7 @@ struct SlabW {
     static constexpr int CHK = MN * Tr::BK / GNT;   // elements per thread per slab
     static constexpr int TPR = Tr::BK / CHK;        // threads per row
     static_assert(CHK % 8 == 0 && TPR * CHK == Tr::BK && MN * TPR == GNT, "slab split");
+    static_assert(KIND != LP_F16, "FP16 W needs its per-vector scale: formed by wprep");
     struct Raw {
         f32x4 pv[CHK / 4], pu[CHK / 4];
     };
@@ -240,6 +254,17 @@ __device__ __forceinline__ void mfma_slab(const SmemV2<KIND, BM, BN>& sm, int cu
 #pragma unroll
                 for (int j = 0; j < FN; ++j)
                     acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][s], bf[j][s], acc.v[i][j], 0, 0, 0);
+    } else if constexpr (KIND == LP_F16) {
+        f16x8 ah[FM], bh[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) ah[i] = *reinterpret_cast<const f16x8*>(&sm.a[cur][0][wm + 16 * i + r16][8 * g]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bh[j] = *reinterpret_cast<const f16x8*>(&sm.b[cur][0][wn + 16 * j + r16][8 * g]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc.v[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc.v[i][j], 0, 0, 0);
     } else {
         bf16x8 ah[FM], bh[FN], alo[Tr::NSPLIT == 2 ? FM : 1], blo[Tr::NSPLIT == 2 ? FN : 1];
 #pragma unroll
@@ -337,6 +362,8 @@ __device__ __forceinline__ void put_v2(typename V2Traits<KIND>::elem* hi, typena
 {
     if constexpr (KIND == LP_F32) {
         hi[i] = v;
+    } else if constexpr (KIND == LP_F16) {
+        hi[i] = f32_to_f16_bits(v);
     } else {
         unsigned short h = f32_to_bf16_bits(v);
         hi[i] = h;

@@ -102,6 +102,7 @@ int gemm_lowp(hipStream_t s, int kind, int m, int n, int k, const double* a, lon
         case LP_F32: return gemm_plain<LP_F32>(s, m, n, k, a, lda, b, ldb, c, ldc);
         case LP_BF16: return gemm_plain<LP_BF16>(s, m, n, k, a, lda, b, ldb, c, ldc);
         case LP_BF16X3: return gemm_plain<LP_BF16X3>(s, m, n, k, a, lda, b, ldb, c, ldc);
+        case LP_F16: return gemm_plain<LP_F16>(s, m, n, k, a, lda, b, ldb, c, ldc);
     }
     return fail(NGW_ERR_ARG, "bad low-precision kind");
 }

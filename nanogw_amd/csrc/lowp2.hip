@@ -2,7 +2,7 @@
 //
 // Same algebra as the FP64 path (shared/apply_matvec_BLOCK.F90:25-208, the
 // operator applied in ISDF-factored form), with the three GEMMs on BF16 /
-// BF16x3 / FP32 MFMA:
+// FP16 / BF16x3 / FP32 MFMA:
 //   per system (once):  pct = (Psi_c / sigma_c)^T  [mu][c]   K1 A operand
 //                       pc  =  Psi_c / sigma_c     [c][mu]   K3 A operand
 //                       m32t = M^T FP32            [mu][nu]  K2 A operand
@@ -17,6 +17,10 @@
 //            (diagonal term, Horner/Clenshaw stage, Lanczos accumulate).
 // All [row][k] operand arrays have k padded with zeros to a multiple of
 // LP_KPAD and one extra all-zero row after the last (see OperandV2).
+// FP16 (range 6e-8 .. 65504) adds exact power-of-two scales: pct and pc carry
+// a factor 2^12; x' is scaled per vector so that its largest entry is at most
+// 2^14 (xscale pass), W per (v, b) by the Psi_v band scale and max |u(:, b)|
+// (taken in K2's epilogue); the K1 and K3 epilogues undo them.
 // Every kernel reads one request descriptor (by value for a single context,
 // from a device table for the grouped batch engine), so molecules of
 // different sizes share a launch; tiles never mix requests.
@@ -56,15 +60,23 @@ struct LpCfg {
     int fw = 0;  // K3 only: 1 = W formed in the slab loader (no wprep pass)
 };
 constexpr int LP_NCFG = 4;
-constexpr LpCfg K1_CFG[2][LP_NCFG] = {
+constexpr LpCfg K1_CFG[3][LP_NCFG] = {
     {{128, 128, 2, 8}, {128, 128, 2, 4}, {128, 128, 2, 1}, {256, 128, 2, 8}},   // FP32, BF16
-    {{128, 128, 2, 8}, {128, 128, 2, 1}, {128, 64, 2, 8}, {128, 128, 1, 8}}};   // BF16x3
-constexpr LpCfg K3_CFG[2][LP_NCFG] = {
+    {{128, 128, 2, 8}, {128, 128, 2, 1}, {128, 64, 2, 8}, {128, 128, 1, 8}},    // BF16x3
+    {{128, 128, 2, 8}, {128, 128, 2, 4}, {128, 128, 2, 1}, {256, 128, 2, 8}}};  // FP16
+constexpr LpCfg K3_CFG[3][LP_NCFG] = {
     {{128, 128, 2, 4}, {128, 128, 2, 4, 1}, {128, 128, 2, 2}, {128, 128, 2, 8, 1}},
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 1}, {128, 128, 2, 1}, {64, 128, 2, 8}}};
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 1}, {128, 128, 2, 1}, {64, 128, 2, 8}},
+    {{128, 128, 2, 4}, {128, 128, 2, 2}, {128, 128, 2, 8}, {128, 128, 2, 1}}};  // FP16: W from wprep
 constexpr int NXCD = 8;
-__host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : 0; }
-static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : 0; }
+__host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : kind == LP_F16 ? 2 : 0; }
+static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : prec == PREC_F16 ? 2 : 0; }
+
+// FP16 operand scales (powers of two, so exact): operator operands x 2^12
+// (|Psi_c / sigma_c| <= 2^0.5), vector operands up to 2^14; a K1 or K3
+// product then carries 2^26.
+constexpr float F16_OPS = 4096.0f, F16_VEC = 16384.0f;
+constexpr double F16_UNSCALE = 1.0 / 67108864.0;
 
 // Workgroups of one request's GEMM: NXCD rectangles of the largest size
 // (workgroups of a smaller rectangle beyond its tiles exit at once).
@@ -105,11 +117,13 @@ __global__ void prep_psi_kernel(const double* psi, long ldp, int nmu, int nv, in
         if (t < n1) {
             const int mu = (int)(t / ldc1), c = (int)(t - (long)mu * ldc1);
             float v = (c < nc && mu < nmu) ? (float)(psi[mu + (long)(nv + c) * ldp] / (double)sig[nv + c]) : 0.0f;
+            if constexpr (KIND == LP_F16) v *= F16_OPS;
             put_v2<KIND>((E*)pct_hi, (E*)pct_lo, t, v);
         } else {
             const long u = t - n1;
             const int c = (int)(u / ldm1), mu = (int)(u - (long)c * ldm1);
             float v = (mu < nmu && c < nc) ? (float)(psi[mu + (long)(nv + c) * ldp] / (double)sig[nv + c]) : 0.0f;
+            if constexpr (KIND == LP_F16) v *= F16_OPS;
             put_v2<KIND>((E*)pc_hi, (E*)pc_lo, u, v);
         }
     }
@@ -131,6 +145,32 @@ __global__ void prep_m_kernel(const double* m, long ldm, int nmu, int ldm1, cons
     }
 }
 
+// Maximum of one value per thread over a 256-thread workgroup (all threads).
+__device__ __forceinline__ float block_max256(float m)
+{
+    __shared__ float part[4];
+    m = fmaxf(m, __shfl_xor(m, 32));
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 8));
+    m = fmaxf(m, __shfl_xor(m, 4));
+    m = fmaxf(m, __shfl_xor(m, 2));
+    m = fmaxf(m, __shfl_xor(m, 1));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    return fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+}
+
+// vsc(v) = power of two >= max_mu |Psi_v(mu)| (one workgroup per band).
+__global__ __launch_bounds__(256) void psiv_scale_kernel(const float* psiv32, int ldm1, int nmu, float* vsc)
+{
+    const float* p = psiv32 + (long)blockIdx.x * ldm1;
+    float m = 0.0f;
+    for (int mu = threadIdx.x; mu < nmu; mu += 256) m = fmaxf(m, fabsf(p[mu]));
+    m = block_max256(m);
+    if (threadIdx.x == 0) vsc[blockIdx.x] = pow2_ceil(m);
+}
+
 static int padk(int n) { return (n + LP_KPAD - 1) / LP_KPAD * LP_KPAD; }
 
 int lp2_prepare(ngw_context* ctx)
@@ -145,6 +185,9 @@ int lp2_prepare(ngw_context* ctx)
     NGW_TRY(ctx->pc32.ensure(4 * n2));
     NGW_TRY(ctx->pctb.ensure(2 * 2 * n1));  // hi then lo
     NGW_TRY(ctx->pcb.ensure(2 * 2 * n2));
+    NGW_TRY(ctx->pcth.ensure(2 * n1));
+    NGW_TRY(ctx->pch.ensure(2 * n2));
+    NGW_TRY(ctx->psiv_scale.ensure(sizeof(float) * nv));
     NGW_TRY(ctx->m32t.ensure(4 * (size_t)(nmu + 1) * ctx->ldm1));
     NGW_TRY(ctx->psiv32.ensure(4 * (size_t)nv * ctx->ldm1));
     hipStream_t s = ctx->stream;
@@ -156,8 +199,12 @@ int lp2_prepare(ngw_context* ctx)
                        ctx->ldc1, ctx->ldm1, (void*)ctx->pct32.ptr, (void*)nullptr, (void*)ctx->pc32.ptr, (void*)nullptr);
     hipLaunchKernelGGL(prep_psi_kernel<LP_BF16X3>, dim3(2048), dim3(256), 0, s, psi, (long)ctx->ld_psi, nmu, nv, nc,
                        sig, ctx->ldc1, ctx->ldm1, (void*)pctb, (void*)(pctb + n1), (void*)pcb, (void*)(pcb + n2));
+    hipLaunchKernelGGL(prep_psi_kernel<LP_F16>, dim3(2048), dim3(256), 0, s, psi, (long)ctx->ld_psi, nmu, nv, nc, sig,
+                       ctx->ldc1, ctx->ldm1, (void*)ctx->pcth.ptr, (void*)nullptr, (void*)ctx->pch.ptr, (void*)nullptr);
     hipLaunchKernelGGL(prep_m_kernel, dim3(2048), dim3(256), 0, s, ctx->mmat.as<double>(), (long)ctx->ld_m, nmu,
                        ctx->ldm1, psi, (long)ctx->ld_psi, nv, ctx->m32t.as<float>(), ctx->psiv32.as<float>());
+    hipLaunchKernelGGL(psiv_scale_kernel, dim3(nv), dim3(256), 0, s, ctx->psiv32.as<float>(), ctx->ldm1, nmu,
+                       ctx->psiv_scale.as<float>());
     NGW_HIP(hipGetLastError());
     NGW_HIP(hipStreamSynchronize(s));
     ctx->lp2_ready = true;
@@ -178,6 +225,37 @@ int lp2_prepare(ngw_context* ctx)
     const int local = pre ? bid - pre[r] : bid;
 
 // ---------------------------------------------------------------- conversion passes
+// FP16: partial maxima of |sigma_c s(v,c) x_b(v,c)| over XS_SPLIT ranges of
+// each vector (XS_SPLIT workgroups per vector, four loads in flight per
+// thread); xprep turns them into the vector's scale xsc(b).
+constexpr int XS_SPLIT = 8;
+__global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
+                                                     int nstage, int s, const int* pre, int nreq)
+{
+    LP_PICK_OP();
+    const int b = local / XS_SPLIT, part = local - b * XS_SPLIT;
+    const double* xb = (ops ? st[r * nstage + s].x : st1.x) + (long)b * op.ld;
+    const long ncv = (long)op.nv * op.nc;
+    const long chunk = (ncv + XS_SPLIT - 1) / XS_SPLIT;
+    const long lo = part * chunk, hi = min(ncv, lo + chunk);
+    double m = 0.0;
+    for (long base = lo + threadIdx.x; base < hi; base += 1024) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long idx = base + 256 * u;
+            v[u] = 0.0;
+            if (idx < hi) {
+                const double xs = op.tda ? xb[idx] : op.sqrtr[idx] * xb[idx];
+                v[u] = fabs(xs) * (double)op.sig[op.nv + (int)(idx / op.nv)];
+            }
+        }
+        m = fmax(m, fmax(fmax(v[0], v[1]), fmax(v[2], v[3])));
+    }
+    const float mb = block_max256((float)m);
+    if (threadIdx.x == 0) op.xpart[local] = mb;
+}
+
 // x'(j = v + nv b, c) = sigma_c s(v,c) x_b(v,c) in (32 v) x (64 c) tiles
 // through LDS: reads run along v (contiguous in x), writes are eight
 // consecutive c per thread in one 16- or 32-byte store.
@@ -198,6 +276,15 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
         for (int c = threadIdx.x; c < op.ldc1; c += 256)
             put_v2<KIND>((E*)op.xlp_hi, (E*)op.xlp_lo, (long)op.nv * op.nvec * op.ldc1 + c, 0.0f);
     const double* xb = x + (long)b * op.ld;
+    double f16s = 1.0;
+    if constexpr (KIND == LP_F16) {
+        float m = 0.0f;
+#pragma unroll
+        for (int k = 0; k < XS_SPLIT; ++k) m = fmaxf(m, op.xpart[b * XS_SPLIT + k]);
+        const float xsc = pow2_ceil(m);
+        f16s = (double)F16_VEC / (double)xsc;
+        if (rem == 0 && threadIdx.x == 0) op.xsc[b] = xsc;  // for K1's epilogue
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int v = v0 + tx, c = c0 + ty + 8 * i;
@@ -206,7 +293,9 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
             const long idx = v + (long)c * op.nv;
             const double xv = xb[idx];
             const double xs = op.tda ? xv : op.sqrtr[idx] * xv;
-            val = (float)(xs * (double)op.sig[op.nv + c]);
+            double sc = (double)op.sig[op.nv + c];
+            if constexpr (KIND == LP_F16) sc *= f16s;
+            val = (float)(xs * sc);
         }
         tile[ty + 8 * i][tx] = val;
     }
@@ -225,7 +314,7 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
         unsigned hi[4], lo[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const unsigned short ha = f32_to_bf16_bits(w[2 * e]), hb = f32_to_bf16_bits(w[2 * e + 1]);
+            const unsigned short ha = to_e16<KIND>(w[2 * e]), hb = to_e16<KIND>(w[2 * e + 1]);
             hi[e] = (unsigned)ha | ((unsigned)hb << 16);
             if constexpr (V2Traits<KIND>::NSPLIT == 2) {
                 const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
@@ -257,6 +346,7 @@ __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, co
     }
     op.t32[i] = out;
     if (b == op.nvec) op.u32[i] = 0.0f;
+    if (op.umax && nu == 0 && b < op.nvec) op.umax[b] = 0u;  // K2 takes the maximum
 }
 
 // ---------------------------------------------------------------- GEMMs
@@ -299,11 +389,14 @@ __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, co
             for (int cc = c0; cc < c1; ++cc) sum += sm.red[row][cc];
             int gm = m0 + pass * HALF + row;
             int slot = tn - (b * nv) / BN;
-            if (gm < op.nmu) op.tpart[((long)slot * op.nvec + b) * op.nmu + gm] = (double)sum;
+            double t = (double)sum;
+            if constexpr (KIND == LP_F16) t *= (double)op.xsc[b] * (double)(1.0f / F16_VEC) * (double)(1.0f / F16_OPS);
+            if (gm < op.nmu) op.tpart[((long)slot * op.nvec + b) * op.nmu + gm] = t;
         }
     }
 }
 
+template <bool UMAX>
 __global__ __launch_bounds__(GNT) void k2v2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
     __shared__ SmemV2<LP_F32, L2K2, L2K2> sm;
@@ -319,6 +412,25 @@ __global__ __launch_bounds__(GNT) void k2v2_kernel(const LpOp* ops, LpOp op1, co
         int gm = m0 + row, gn = n0 + col;
         if (gm < op.ldm1 && gn < op.nvec) op.u32[gm + (long)gn * op.ldm1] = val;
     });
+    if constexpr (UMAX) {
+        // umax(b) = max_mu |u(mu, b)|: per column over the wave's rows (lanes
+        // l, l + 16, l + 32, l + 48 hold one column), then one atomic per column
+        constexpr int FM = L2K2 / 32, FN = L2K2 / 32;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int wn = (wave >> 1) * (L2K2 / 2);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            float m = 0.0f;
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) m = fmaxf(m, fabsf(acc.v[i][j][q]));
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            const int gn = n0 + wn + 16 * j + (lane & 15);
+            if (lane < 16 && gn < op.nvec) atomicMax(op.umax + gn, __float_as_uint(m));
+        }
+    }
 }
 
 // W(j = v + nv b, mu) = Psi_v(mu, v) u(mu, b) for rows j <= N (row N zero),
@@ -339,6 +451,13 @@ __global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, c
         const float* u = op.u32 + (long)b * op.ldm1 + mu;
         w0 = gload<f32x4>(p) * gload<f32x4>(u);
         w1 = gload<f32x4>(p + 4) * gload<f32x4>(u + 4);
+        if constexpr (KIND == LP_F16) {
+            const float sc = op.vsc[v] * pow2_ceil(__uint_as_float(op.umax[b]));
+            const float f = F16_VEC / sc;
+            w0 = w0 * f;
+            w1 = w1 * f;
+            if (mu == 0) op.wsc[j] = sc;
+        }
     }
     const long o = (long)j * op.ldm1 + mu;
     if constexpr (KIND == LP_F32) {
@@ -350,7 +469,7 @@ __global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, c
         unsigned hi[4], lo[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const unsigned short ha = f32_to_bf16_bits(w[2 * e]), hb = f32_to_bf16_bits(w[2 * e + 1]);
+            const unsigned short ha = to_e16<KIND>(w[2 * e]), hb = to_e16<KIND>(w[2 * e + 1]);
             hi[e] = (unsigned)ha | ((unsigned)hb << 16);
             if constexpr (V2Traits<KIND>::NSPLIT == 2) {
                 const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
@@ -405,7 +524,9 @@ __global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, co
         int c = m0 + row, j = n0 + col;
         if (c < op.nc && j < N) {
             int b = j / nv, v = j - b * nv;
-            ep.apply(v + (long)c * nv, b, (double)val * (double)sigc[c]);
+            double y = (double)val * (double)sigc[c];
+            if constexpr (KIND == LP_F16) y *= (double)op.wsc[j] * F16_UNSCALE;
+            ep.apply(v + (long)c * nv, b, y);
         }
     });
 }
@@ -413,13 +534,14 @@ __global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, co
 // ---------------------------------------------------------------- launch plumbing
 // Number of workgroups of each kernel for one request.
 struct LpGrid {
-    int xprep, k1, red, k2, wprep, k3;
+    int xscale, xprep, k1, red, k2, wprep, k3;
 };
 
 static LpGrid lp_grid(int prec, int c1, int c3, int nmu, int nv, int nc, int nvec, int ldc1, int ldm1)
 {
     const LpCfg k1 = K1_CFG[cfg_set_prec(prec)][c1], k3 = K3_CFG[cfg_set_prec(prec)][c3];
     LpGrid g;
+    g.xscale = prec == PREC_F16 ? nvec * XS_SPLIT : 0;
     g.xprep = nvec * ((nv + 31) / 32) * (ldc1 / 64);
     g.k1 = xcd_grid((nmu + k1.bm - 1) / k1.bm, (nv * nvec + k1.bn - 1) / k1.bn, k1.pm);
     g.red = (int)(((long)(nvec + 1) * ldm1 + 255) / 256);
@@ -450,40 +572,44 @@ static int lp_launch(hipStream_t st, int c1, int c3, const LpOp* ops, const LpOp
                      const LpStage& st1, int nstage, int s, const int* const* pre, const int* tot, int nreq,
                      ngw_context* prof)
 {
+    if constexpr (KIND == LP_F16) {
+        ProfScope ps(prof, KC_CONVERT);
+        hipLaunchKernelGGL(xscale_kernel, dim3(tot[0]), dim3(256), 0, st, ops, op1, stg, st1, nstage, s, pre[0], nreq);
+    }
     {
         ProfScope ps(prof, KC_CONVERT);
-        hipLaunchKernelGGL(xprep_kernel<KIND>, dim3(tot[0]), dim3(256), 0, st, ops, op1, stg, st1, nstage, s, pre[0],
+        hipLaunchKernelGGL(xprep_kernel<KIND>, dim3(tot[1]), dim3(256), 0, st, ops, op1, stg, st1, nstage, s, pre[1],
                            nreq);
     }
     {
         ProfScope ps(prof, KC_K1);
         switch (c1) {
-            case 0: launch_k1<KIND, 0>(st, tot[1], ops, op1, pre[1], nreq); break;
-            case 1: launch_k1<KIND, 1>(st, tot[1], ops, op1, pre[1], nreq); break;
-            case 2: launch_k1<KIND, 2>(st, tot[1], ops, op1, pre[1], nreq); break;
-            default: launch_k1<KIND, 3>(st, tot[1], ops, op1, pre[1], nreq); break;
+            case 0: launch_k1<KIND, 0>(st, tot[2], ops, op1, pre[2], nreq); break;
+            case 1: launch_k1<KIND, 1>(st, tot[2], ops, op1, pre[2], nreq); break;
+            case 2: launch_k1<KIND, 2>(st, tot[2], ops, op1, pre[2], nreq); break;
+            default: launch_k1<KIND, 3>(st, tot[2], ops, op1, pre[2], nreq); break;
         }
     }
     {
         ProfScope ps(prof, KC_CONVERT);
-        hipLaunchKernelGGL(red2_kernel, dim3(tot[2]), dim3(256), 0, st, ops, op1, pre[2], nreq,
+        hipLaunchKernelGGL(red2_kernel, dim3(tot[3]), dim3(256), 0, st, ops, op1, pre[3], nreq,
                            K1_CFG[cfg_set(KIND)][c1].bn);
     }
     {
         ProfScope ps(prof, KC_K2);
-        hipLaunchKernelGGL(k2v2_kernel, dim3(tot[3]), dim3(GNT), 0, st, ops, op1, pre[3], nreq);
+        hipLaunchKernelGGL(k2v2_kernel<KIND == LP_F16>, dim3(tot[4]), dim3(GNT), 0, st, ops, op1, pre[4], nreq);
     }
     if (!K3_CFG[cfg_set(KIND)][c3].fw) {
         ProfScope ps(prof, KC_CONVERT);
-        hipLaunchKernelGGL(wprep_kernel<KIND>, dim3(tot[4]), dim3(256), 0, st, ops, op1, pre[4], nreq);
+        hipLaunchKernelGGL(wprep_kernel<KIND>, dim3(tot[5]), dim3(256), 0, st, ops, op1, pre[5], nreq);
     }
     {
         ProfScope ps(prof, KC_K3);
         switch (c3) {
-            case 0: launch_k3<KIND, 0>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
-            case 1: launch_k3<KIND, 1>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
-            case 2: launch_k3<KIND, 2>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
-            default: launch_k3<KIND, 3>(st, tot[5], ops, op1, stg, st1, nstage, s, pre[5], nreq); break;
+            case 0: launch_k3<KIND, 0>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
+            case 1: launch_k3<KIND, 1>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
+            case 2: launch_k3<KIND, 2>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
+            default: launch_k3<KIND, 3>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
         }
     }
     NGW_HIP(hipGetLastError());
@@ -500,6 +626,7 @@ int lp2_launch_any(int kind, hipStream_t st, int c1, int c3, const LpOp* ops, co
         case PREC_BF16: return lp_launch<LP_BF16>(st, c1, c3, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
         case PREC_BF16X3:
             return lp_launch<LP_BF16X3>(st, c1, c3, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
+        case PREC_F16: return lp_launch<LP_F16>(st, c1, c3, ops, op1, stg, st1, nstage, s, pre, tot, nreq, prof);
     }
     return fail(NGW_ERR_STATE, "unknown precision");
 }
@@ -508,8 +635,9 @@ int lp2_launch_any(int kind, hipStream_t st, int c1, int c3, const LpOp* ops, co
 int lp2_slots(int nv) { return std::max(k1_slots(nv), (nv - 1) / K1_BN_MIN + 2); }
 
 // Bytes of the per-request low-precision workspaces: x' ([j][c]) and W
-// ([j][mu]) in the MFMA type, t and u (FP32, [b][mu]); operand arrays carry
-// their zero row; each part starts 256-byte aligned.
+// ([j][mu]) in the MFMA type, t and u (FP32, [b][mu]), the FP16 per-vector
+// scales; operand arrays carry their zero row; each part starts 256-byte
+// aligned.
 static size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
 size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec)
@@ -517,7 +645,8 @@ size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec)
     const size_t es = kind == PREC_F32 ? 4 : 2, ns = kind == PREC_BF16X3 ? 2 : 1;
     const size_t ldc1 = padk(nc), ldm1 = padk(nmu), N = (size_t)nv * nvec;
     return ns * (align256(es * (N + 1) * ldc1) + align256(es * (N + 1) * ldm1)) +
-           2 * align256(4 * (size_t)(nvec + 1) * ldm1);
+           2 * align256(4 * (size_t)(nvec + 1) * ldm1) + 2 * align256(4 * (size_t)nvec) + align256(4 * N) +
+           align256(4 * (size_t)nvec * XS_SPLIT);
 }
 
 // Fill the descriptor of one request from its operator and workspaces.
@@ -528,6 +657,9 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
     if (kind == PREC_F32) {
         d.pct_hi = op->pct32.ptr; d.pct_lo = nullptr;
         d.pc_hi = op->pc32.ptr; d.pc_lo = nullptr;
+    } else if (kind == PREC_F16) {
+        d.pct_hi = op->pcth.ptr; d.pct_lo = nullptr;
+        d.pc_hi = op->pch.ptr; d.pc_lo = nullptr;
     } else {
         d.pct_hi = op->pctb.ptr; d.pct_lo = op->pctb.as<unsigned short>() + n1;
         d.pc_hi = op->pcb.ptr; d.pc_lo = op->pcb.as<unsigned short>() + n2;
@@ -550,6 +682,13 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
     char* t0 = w0 + wsz * ns;
     d.t32 = reinterpret_cast<float*>(t0);
     d.u32 = reinterpret_cast<float*>(t0 + tsz);
+    char* s0 = t0 + 2 * tsz;
+    const bool f16 = kind == PREC_F16;
+    d.xsc = f16 ? reinterpret_cast<float*>(s0) : nullptr;
+    d.umax = f16 ? reinterpret_cast<unsigned*>(s0 + align256(4 * (size_t)nvec)) : nullptr;
+    d.vsc = f16 ? op->psiv_scale.as<float>() : nullptr;
+    d.wsc = f16 ? reinterpret_cast<float*>(s0 + 2 * align256(4 * (size_t)nvec)) : nullptr;
+    d.xpart = f16 ? reinterpret_cast<float*>(s0 + 2 * align256(4 * (size_t)nvec) + align256(4 * N)) : nullptr;
     d.tpart = tpart;
     d.q = nullptr; d.w = nullptr; d.a = nullptr; d.b = nullptr; d.niter = 0; d.kstep = nullptr;
 }
@@ -557,7 +696,8 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
 void lp2_grid_counts(int kind, int c1, int c3, const LpOp& d, int* tot)
 {
     LpGrid g = lp_grid(kind, c1, c3, d.nmu, d.nv, d.nc, d.nvec, d.ldc1, d.ldm1);
-    tot[0] = g.xprep; tot[1] = g.k1; tot[2] = g.red; tot[3] = g.k2; tot[4] = g.wprep; tot[5] = g.k3;
+    tot[0] = g.xscale; tot[1] = g.xprep; tot[2] = g.k1; tot[3] = g.red; tot[4] = g.k2; tot[5] = g.wprep;
+    tot[6] = g.k3;
 }
 
 // Single-context operator application in a low-precision mode.
@@ -579,7 +719,7 @@ int operator_lp2(ngw_context* ctx, const OperatorArgs& a)
     const int c1 = ctx->lp_cfg[kind][0], c3 = ctx->lp_cfg[kind][1];
     int tot[LP_NKERNEL];
     lp2_grid_counts(kind, c1, c3, d, tot);
-    const int* pre[LP_NKERNEL] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const int* pre[LP_NKERNEL] = {};
     return lp2_launch_any(kind, ctx->stream, c1, c3, nullptr, d, nullptr, st, 1, 0, pre, tot, 1, ctx);
 }
 

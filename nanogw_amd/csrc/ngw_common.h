@@ -58,7 +58,7 @@ struct DevBuf {
 enum KernelClass { KC_K1 = 0, KC_K2, KC_K3, KC_LANCZOS, KC_CFRAC, KC_PVEC, KC_SIGMA, KC_CONVERT, KC_COUNT };
 
 // Precision of the three GEMM stages of the operator.
-enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2, PREC_BF16X3 = 3 };
+enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2, PREC_BF16X3 = 3, PREC_F16 = 4, PREC_COUNT = 5 };
 
 }  // namespace ngw
 
@@ -87,9 +87,9 @@ struct ngw_context {
     // pre-converted operands of the streamlined low-precision path
     bool lp2_ready = false;
     int ldc1 = 0, ldm1 = 0;
-    ngw::DevBuf pct32, pc32, pctb, pcb, m32t, psiv32, lpwork;
+    ngw::DevBuf pct32, pc32, pctb, pcb, pcth, pch, m32t, psiv32, psiv_scale, lpwork;
     // chosen {K1, K3} tile configuration per precision mode (ngw_autotune)
-    int lp_cfg[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    int lp_cfg[ngw::PREC_COUNT][2] = {};
     // workspaces
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;
     // host staging for drop-in calls

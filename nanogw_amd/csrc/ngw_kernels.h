@@ -115,6 +115,11 @@ struct LpOp {
     long ld; int tda;
     void* xlp_hi; void* xlp_lo; void* wlp_hi; void* wlp_lo;
     float* t32; float* u32; double* tpart;
+    // FP16 only: per-vector power-of-two scales of x' (xsc, from the partial
+    // maxima xpart) and of u (from umax = max |u(:, b)| as float bits),
+    // per-band scales of Psi_v (vsc), scale of W row j = v + nv b
+    // (wsc = vsc(v) x scale of u(:, b))
+    float* xsc; unsigned* umax; const float* vsc; float* wsc; float* xpart;
     double* q; double* w; double* a; double* b; int niter; int* kstep;
 };
 
@@ -133,9 +138,10 @@ int lp2_launch_any(int kind, hipStream_t st, int c1, int c3, const LpOp* ops, co
 int lp2_slots(int nv);
 size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec);
 void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char* work, double* tpart, LpOp& d);
-// Kernels per low-precision operator application (xprep, K1, slot reduction,
-// K2, wprep, K3): the number of workgroup prefix tables a grouped launch needs.
-constexpr int LP_NKERNEL = 6;
+// Kernels per low-precision operator application (x' scale (FP16 only),
+// xprep, K1, slot reduction, K2, wprep, K3): the number of workgroup prefix
+// tables a grouped launch needs.
+constexpr int LP_NKERNEL = 7;
 void lp2_grid_counts(int kind, int c1, int c3, const LpOp& d, int* tot);
 int operator_lp2(ngw_context* ctx, const OperatorArgs& a);
 

@@ -12,7 +12,7 @@ module ngw_c
 
   integer(c_int), parameter :: NGW_OK = 0
   integer(c_int), parameter :: NGW_PREC_F64 = 0, NGW_PREC_F32 = 1, &
-                               NGW_PREC_BF16 = 2, NGW_PREC_BF16X3 = 3
+                               NGW_PREC_BF16 = 2, NGW_PREC_BF16X3 = 3, NGW_PREC_F16 = 4
 
   interface
 

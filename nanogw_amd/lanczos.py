@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 
-PREC = {"f64": 0, "f32": 1, "bf16": 2, "bf16x3": 3}
+PREC = {"f64": 0, "f32": 1, "bf16": 2, "bf16x3": 3, "f16": 4}
 
 
 def _f64(a):
