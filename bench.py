@@ -286,7 +286,7 @@ def run_stream(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     recs, st = serve_stream(ctx, mlist, reqs, niter, nen, deltae, ecuts, budget, sigx=sigx,
-                            use_graph=not args.no_graph, batch=batch, sids=sids, polys=polys)
+                            use_graph=not args.no_graph, batch=batch, sids=sids, polys=polys, log=log)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     solves = sum(len(r["qp"]) for r in reqs)
@@ -517,8 +517,11 @@ def main():
         "data": "synthetic (seeded Gaussian-Hermite molecule, CVT ISDF points, device V build)",
         "config": {"workload": f"{args.config}: R={mol.radius} h={mol.h} ngrid={mol.ngrid} nv={nv} nc={nc} "
                                f"Nmu={nmu} N_b={nb} niter={niter} npoly={npoly} nen={nen}, zeta ridge "
-                               f"{mol.ridge:g}; {args.mode} mode, "
-                               f"one QP state per step" + (", hipGraph replay" if args.graph else ""),
+                               f"{mol.ridge:g}; " +
+                               ("served mode: the timed QP states are requests of one continuous-batching "
+                                "loop (all N_b bands of every state in each launch)" if args.mode == "served"
+                                else "single mode: one ngw_sigma_lanczos call per QP state") +
+                               (", hipGraph replay" if args.graph else ""),
                    "qp_states": states[args.warmup:], "parallelism": "single GPU",
                    "tile_configs": tiles},
         "eqp_ev": {str(m): e for m, e in results.items()},

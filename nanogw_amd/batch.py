@@ -126,6 +126,8 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
     t0 = time.perf_counter()
     pi = 0
     steps = 0
+    done = 0
+    t_log = t0
     while pi < len(pending) or queue or running:
         now = time.perf_counter() - t0
         while pi < len(pending) and requests[pending[pi]]["t"] <= now:
@@ -149,6 +151,10 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
             continue
         batch.step(1, use_graph)
         steps += 1
+        if log and time.perf_counter() - t_log > 20.0:
+            t_log = time.perf_counter()
+            log(f"[serve] {t_log - t0:.1f}s: {steps} batched steps, {len(running)} running, {len(queue)} queued, "
+                f"{done} done")
         for rid in list(running):
             i, need = running[rid]
             sig = batch.result(rid, nen, len(requests[i]["qp"]))
@@ -164,6 +170,7 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
             batch.release(rid)
             del running[rid]
             used -= need
+            done += 1
             if log:
                 log(f"[serve] request {i} done: latency {recs[i]['finish'] - recs[i]['arrival']:.3f}s")
     stats = batch.stats()
