@@ -53,11 +53,11 @@ template <int KIND>
 __global__ __launch_bounds__(GNT) void gemm_plain_v2_kernel(OperandV2<KIND> A, OperandV2<KIND> B, int m, int n,
                                                             int kpad, double* c, long ldc, int mtiles)
 {
-    __shared__ SmemV2<KIND, 64, 64> sm;
+    __shared__ SmemV2<KIND, 64, 64, 1> sm;
     const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int m0 = (tile % mtiles) * 64, n0 = (tile / mtiles) * 64;
     AccLp<64, 64> acc;
-    gemm_v2_tile<KIND, 64, 64, 2>(A, B, m0, n0, kpad, acc, sm);
+    gemm_v2_tile<KIND, 64, 64, 2>(A, B, m0, n0, kpad, acc, sm);  // swizzled LDS rows
     for_acc_lp<64, 64>(acc, [&](int row, int col, float val) {
         int gm = m0 + row, gn = n0 + col;
         if (gm < m && gn < n) c[gm + (long)gn * ldc] = (double)val;

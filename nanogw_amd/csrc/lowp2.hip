@@ -58,16 +58,17 @@ constexpr int L2K2 = 64;
 struct LpCfg {
     int bm, bn, d, pm;
     int fw = 0;  // K3 only: 1 = W formed in the slab loader (no wprep pass)
+    int sw = 0;  // LDS layout (lds_chunk): 0 padded rows, 1 XOR-swizzled rows
 };
 constexpr int LP_NCFG = 4;
 constexpr LpCfg K1_CFG[3][LP_NCFG] = {
-    {{128, 128, 2, 8}, {128, 128, 2, 4}, {128, 128, 2, 1}, {256, 128, 2, 8}},   // FP32, BF16
-    {{128, 128, 2, 8}, {128, 128, 2, 1}, {128, 64, 2, 8}, {128, 128, 1, 8}},    // BF16x3
-    {{128, 128, 2, 8}, {128, 128, 2, 4}, {128, 128, 2, 1}, {256, 128, 2, 8}}};  // FP16
+    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 128, 2, 4, 0, 1}, {256, 128, 2, 8, 0, 1}},  // FP32, BF16
+    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 64, 2, 8, 0, 1}, {128, 128, 1, 8, 0, 1}},   // BF16x3
+    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 128, 2, 4, 0, 1}, {256, 128, 2, 8, 0, 1}}}; // FP16
 constexpr LpCfg K3_CFG[3][LP_NCFG] = {
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 1}, {128, 128, 2, 2}, {128, 128, 2, 8, 1}},
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 1}, {128, 128, 2, 1}, {64, 128, 2, 8}},
-    {{128, 128, 2, 4}, {128, 128, 2, 2}, {128, 128, 2, 8}, {128, 128, 2, 1}}};  // FP16: W from wprep
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {128, 128, 2, 8, 1, 1}},
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {64, 128, 2, 8}},
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 8, 0, 1}, {128, 128, 3, 4}}};  // FP16: W from wprep
 constexpr int NXCD = 8;
 __host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : kind == LP_F16 ? 2 : 0; }
 static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : prec == PREC_F16 ? 2 : 0; }
@@ -161,6 +162,16 @@ __device__ __forceinline__ float block_max256(float m)
     return fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
 }
 
+// xsig(v + nv c) = sigma_c sqrtR(v, c) (full) and sigma_c (TDA, second half).
+__global__ void xsig_kernel(const double* sqrtr, const float* sig, int nv, long ncv, float* xsig)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ncv; i += (long)gridDim.x * blockDim.x) {
+        const double sc = (double)sig[nv + (int)(i / nv)];
+        xsig[i] = (float)(sqrtr[i] * sc);
+        xsig[ncv + i] = (float)sc;
+    }
+}
+
 // vsc(v) = power of two >= max_mu |Psi_v(mu)| (one workgroup per band).
 __global__ __launch_bounds__(256) void psiv_scale_kernel(const float* psiv32, int ldm1, int nmu, float* vsc)
 {
@@ -188,6 +199,7 @@ int lp2_prepare(ngw_context* ctx)
     NGW_TRY(ctx->pcth.ensure(2 * n1));
     NGW_TRY(ctx->pch.ensure(2 * n2));
     NGW_TRY(ctx->psiv_scale.ensure(sizeof(float) * nv));
+    NGW_TRY(ctx->xsig.ensure(sizeof(float) * 2 * (size_t)ctx->ncv));
     NGW_TRY(ctx->m32t.ensure(4 * (size_t)(nmu + 1) * ctx->ldm1));
     NGW_TRY(ctx->psiv32.ensure(4 * (size_t)nv * ctx->ldm1));
     hipStream_t s = ctx->stream;
@@ -205,6 +217,8 @@ int lp2_prepare(ngw_context* ctx)
                        ctx->ldm1, psi, (long)ctx->ld_psi, nv, ctx->m32t.as<float>(), ctx->psiv32.as<float>());
     hipLaunchKernelGGL(psiv_scale_kernel, dim3(nv), dim3(256), 0, s, ctx->psiv32.as<float>(), ctx->ldm1, nmu,
                        ctx->psiv_scale.as<float>());
+    hipLaunchKernelGGL(xsig_kernel, dim3(1024), dim3(256), 0, s, ctx->sqrtr.as<double>(), sig, nv, ctx->ncv,
+                       ctx->xsig.as<float>());
     NGW_HIP(hipGetLastError());
     NGW_HIP(hipStreamSynchronize(s));
     ctx->lp2_ready = true;
@@ -225,8 +239,8 @@ int lp2_prepare(ngw_context* ctx)
     const int local = pre ? bid - pre[r] : bid;
 
 // ---------------------------------------------------------------- conversion passes
-// FP16: partial maxima of |sigma_c s(v,c) x_b(v,c)| over XS_SPLIT ranges of
-// each vector (XS_SPLIT workgroups per vector, four loads in flight per
+// FP16: partial maxima of |x'| = |xsig(v,c) x_b(v,c)| over XS_SPLIT ranges
+// of each vector (XS_SPLIT workgroups per vector, eight loads in flight per
 // thread); xprep turns them into the vector's scale xsc(b).
 constexpr int XS_SPLIT = 8;
 __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
@@ -239,39 +253,39 @@ __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, 
     const long chunk = (ncv + XS_SPLIT - 1) / XS_SPLIT;
     const long lo = part * chunk, hi = min(ncv, lo + chunk);
     double m = 0.0;
-    for (long base = lo + threadIdx.x; base < hi; base += 1024) {
-        double v[4];
+    for (long base = lo + threadIdx.x; base < hi; base += 2048) {
+        double v[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const long idx = base + 256 * u;
-            v[u] = 0.0;
-            if (idx < hi) {
-                const double xs = op.tda ? xb[idx] : op.sqrtr[idx] * xb[idx];
-                v[u] = fabs(xs) * (double)op.sig[op.nv + (int)(idx / op.nv)];
-            }
+            v[u] = idx < hi ? fabs(xb[idx]) * (double)op.xsig[idx] : 0.0;
         }
-        m = fmax(m, fmax(fmax(v[0], v[1]), fmax(v[2], v[3])));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
     }
     const float mb = block_max256((float)m);
     if (threadIdx.x == 0) op.xpart[local] = mb;
 }
 
-// x'(j = v + nv b, c) = sigma_c s(v,c) x_b(v,c) in (32 v) x (64 c) tiles
-// through LDS: reads run along v (contiguous in x), writes are eight
-// consecutive c per thread in one 16- or 32-byte store.
+// x'(j = v + nv b, c) = xsig(v,c) x_b(v,c), xsig = sigma_c s(v,c) (sigma_c
+// for TDA), per workgroup one vector's (64 c) x (up to 128 v) block: the reads
+// are one contiguous run of x (all of v for 64 consecutive c), the block is
+// transposed through LDS, and each thread writes eight consecutive c in one
+// 16- or 32-byte store.
+constexpr int XP_V = 128, XP_C = 64;
 template <int KIND>
 __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                     int nstage, int s, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
-    __shared__ float tile[64][33];
+    __shared__ float tile[XP_C][XP_V + 1];
     LP_PICK_OP();
     const double* x = ops ? st[r * nstage + s].x : st1.x;
-    const int nvt = (op.nv + 31) / 32, nct = op.ldc1 / 64;
+    const int nvt = (op.nv + XP_V - 1) / XP_V, nct = op.ldc1 / XP_C;
     const int b = local / (nvt * nct);
     const int rem = local - b * nvt * nct;
-    const int v0 = (rem / nct) * 32, c0 = (rem % nct) * 64;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int v0 = (rem / nct) * XP_V, c0 = (rem % nct) * XP_C;
+    const int vn = min(XP_V, op.nv - v0), cn = max(0, min(XP_C, op.nc - c0));
     if (local == 0)  // the zero row after the last x' row
         for (int c = threadIdx.x; c < op.ldc1; c += 256)
             put_v2<KIND>((E*)op.xlp_hi, (E*)op.xlp_lo, (long)op.nv * op.nvec * op.ldc1 + c, 0.0f);
@@ -285,46 +299,54 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
         f16s = (double)F16_VEC / (double)xsc;
         if (rem == 0 && threadIdx.x == 0) op.xsc[b] = xsc;  // for K1's epilogue
     }
+    // lanes along v (128 per row, rows of one vector are nv apart in x), two
+    // rows per pass, eight rows in flight per thread
+    const int tx = threadIdx.x & (XP_V - 1), ty = threadIdx.x / XP_V;
+    const long base = v0 + tx + (long)c0 * op.nv;
+    for (int cb = 0; cb < cn; cb += 16) {
+        double xv[8], sv[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int v = v0 + tx, c = c0 + ty + 8 * i;
-        float val = 0.0f;
-        if (v < op.nv && c < op.nc) {
-            const long idx = v + (long)c * op.nv;
-            const double xv = xb[idx];
-            const double xs = op.tda ? xv : op.sqrtr[idx] * xv;
-            double sc = (double)op.sig[op.nv + c];
-            if constexpr (KIND == LP_F16) sc *= f16s;
-            val = (float)(xs * sc);
+        for (int u = 0; u < 8; ++u) {
+            const int c = cb + ty + 2 * u;
+            const bool in = tx < vn && c < cn;
+            const long idx = base + (long)c * op.nv;
+            xv[u] = in ? xb[idx] : 0.0;
+            sv[u] = in ? (double)op.xsig[idx] : 0.0;
         }
-        tile[ty + 8 * i][tx] = val;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int c = cb + ty + 2 * u;
+            if (c < XP_C) tile[c][tx] = (float)(xv[u] * (sv[u] * f16s));
+        }
     }
+    for (int c = ((cn + 15) / 16) * 16 + ty; c < XP_C; c += 2) tile[c][tx] = 0.0f;  // k padding past the last pass
     __syncthreads();
-    const int v = v0 + (threadIdx.x >> 3), cq = (threadIdx.x & 7) * 8;
-    if (v >= op.nv) return;
-    float w[8];
+    for (int t = threadIdx.x; t < vn * 8; t += 256) {
+        const int vl = t >> 3, cq = (t & 7) * 8;
+        float w[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) w[e] = tile[cq + e][threadIdx.x >> 3];
-    const long o = (long)(v + op.nv * b) * op.ldc1 + c0 + cq;
-    if constexpr (KIND == LP_F32) {
-        f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.xlp_hi) + o);
-        out[0] = f32x4{w[0], w[1], w[2], w[3]};
-        out[1] = f32x4{w[4], w[5], w[6], w[7]};
-    } else {
-        unsigned hi[4], lo[4];
+        for (int e = 0; e < 8; ++e) w[e] = tile[cq + e][vl];
+        const long o = (long)(v0 + vl + op.nv * b) * op.ldc1 + c0 + cq;
+        if constexpr (KIND == LP_F32) {
+            f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.xlp_hi) + o);
+            out[0] = f32x4{w[0], w[1], w[2], w[3]};
+            out[1] = f32x4{w[4], w[5], w[6], w[7]};
+        } else {
+            unsigned hi[4], lo[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const unsigned short ha = to_e16<KIND>(w[2 * e]), hb = to_e16<KIND>(w[2 * e + 1]);
-            hi[e] = (unsigned)ha | ((unsigned)hb << 16);
-            if constexpr (V2Traits<KIND>::NSPLIT == 2) {
-                const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
-                const unsigned short lb = f32_to_bf16_bits(w[2 * e + 1] - bf16_bits_to_f32(hb));
-                lo[e] = (unsigned)la | ((unsigned)lb << 16);
+            for (int e = 0; e < 4; ++e) {
+                const unsigned short ha = to_e16<KIND>(w[2 * e]), hb = to_e16<KIND>(w[2 * e + 1]);
+                hi[e] = (unsigned)ha | ((unsigned)hb << 16);
+                if constexpr (V2Traits<KIND>::NSPLIT == 2) {
+                    const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
+                    const unsigned short lb = f32_to_bf16_bits(w[2 * e + 1] - bf16_bits_to_f32(hb));
+                    lo[e] = (unsigned)la | ((unsigned)lb << 16);
+                }
             }
+            *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.xlp_hi) + o) = u32x4{hi[0], hi[1], hi[2], hi[3]};
+            if constexpr (V2Traits<KIND>::NSPLIT == 2)
+                *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.xlp_lo) + o) = u32x4{lo[0], lo[1], lo[2], lo[3]};
         }
-        *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.xlp_hi) + o) = u32x4{hi[0], hi[1], hi[2], hi[3]};
-        if constexpr (V2Traits<KIND>::NSPLIT == 2)
-            *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.xlp_lo) + o) = u32x4{lo[0], lo[1], lo[2], lo[3]};
     }
 }
 
@@ -350,13 +372,13 @@ __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, co
 }
 
 // ---------------------------------------------------------------- GEMMs
-template <int KIND, int BM, int BN, int D, int PM>
+template <int KIND, int BM, int BN, int D, int PM, int SW>
 __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
     constexpr int HALF = BM < 64 ? BM : 64;  // rows reduced per LDS pass (keeps the buffer within the slabs' LDS)
     __shared__ union {
-        SmemV2<KIND, BM, BN> ab;
+        SmemV2<KIND, BM, BN, SW> ab;
         float red[HALF][BN + 1];
     } sm;
     LP_PICK_OP();
@@ -399,12 +421,12 @@ __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, co
 template <bool UMAX>
 __global__ __launch_bounds__(GNT) void k2v2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
-    __shared__ SmemV2<LP_F32, L2K2, L2K2> sm;
+    __shared__ SmemV2<LP_F32, L2K2, L2K2, 1> sm;
     LP_PICK_TILE();
     const int mt = (op.nmu + L2K2 - 1) / L2K2;
     const int m0 = (local % mt) * L2K2, n0 = (local / mt) * L2K2;
     AccLp<L2K2, L2K2> acc;
-    gemm_v2_tile<LP_F32, L2K2, L2K2>(OperandV2<LP_F32>{op.m32t, nullptr, op.ldm1, op.nmu},
+    gemm_v2_tile<LP_F32, L2K2, L2K2, 1>(OperandV2<LP_F32>{op.m32t, nullptr, op.ldm1, op.nmu},
                                      OperandV2<LP_F32>{op.t32, nullptr, op.ldm1, op.nvec}, m0, n0, op.ldm1, acc, sm);
     // rows nmu <= mu < ldm1 come out as exact zeros (zero A rows): the K3
     // loader reads u over the whole padded slab
@@ -434,61 +456,88 @@ __global__ __launch_bounds__(GNT) void k2v2_kernel(const LpOp* ops, LpOp op1, co
 }
 
 // W(j = v + nv b, mu) = Psi_v(mu, v) u(mu, b) for rows j <= N (row N zero),
-// eight consecutive mu per thread from the FP32 psiv32 / u rows.
+// in chunks of eight consecutive mu from the FP32 psiv32 / u rows; a
+// workgroup covers WP_CH consecutive chunks, each thread WP_CH / 256 of them
+// 256 chunks apart, with all loads issued before the first store.
+constexpr int WP_CH = 1024;
 template <int KIND>
 __global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
+    constexpr int Q = WP_CH / 256;
     LP_PICK_OP();
     const int per_row = op.ldm1 / 8, N = op.nv * op.nvec;
-    const long i = (long)local * 256 + threadIdx.x;
-    if (i >= (long)(N + 1) * per_row) return;
-    const int j = (int)(i / per_row), mu = (int)(i - (long)j * per_row) * 8;
-    f32x4 w0 = {0.f, 0.f, 0.f, 0.f}, w1 = w0;
-    if (j < N) {
-        const int b = j / op.nv, v = j - b * op.nv;
-        const float* p = op.psiv32 + (long)v * op.ldm1 + mu;
-        const float* u = op.u32 + (long)b * op.ldm1 + mu;
-        w0 = gload<f32x4>(p) * gload<f32x4>(u);
-        w1 = gload<f32x4>(p + 4) * gload<f32x4>(u + 4);
-        if constexpr (KIND == LP_F16) {
-            const float sc = op.vsc[v] * pow2_ceil(__uint_as_float(op.umax[b]));
-            const float f = F16_VEC / sc;
-            w0 = w0 * f;
-            w1 = w1 * f;
-            if (mu == 0) op.wsc[j] = sc;
+    const long nchunk = (long)(N + 1) * per_row;
+    const bool small = nchunk < (1L << 31);  // 32-bit index division (64-bit division is a long VALU sequence)
+    f32x4 w0[Q], w1[Q];
+    float sc[Q];
+    int jq[Q], muq[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const long i = (long)local * WP_CH + threadIdx.x + 256 * q;
+        int j;
+        if (small) j = (int)((unsigned)i / (unsigned)per_row);
+        else j = (int)(i / per_row);
+        const int mu = (int)(i - (long)j * per_row) * 8;
+        jq[q] = j;
+        muq[q] = mu;
+        w0[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+        w1[q] = w0[q];
+        sc[q] = 1.0f;
+        if (i < nchunk && j < N) {
+            const int b = j / op.nv, v = j - b * op.nv;
+            const float* p = op.psiv32 + (long)v * op.ldm1 + mu;
+            const float* u = op.u32 + (long)b * op.ldm1 + mu;
+            w0[q] = gload<f32x4>(p) * gload<f32x4>(u);
+            w1[q] = gload<f32x4>(p + 4) * gload<f32x4>(u + 4);
+            if constexpr (KIND == LP_F16) sc[q] = op.vsc[v] * pow2_ceil(__uint_as_float(op.umax[b]));
         }
     }
-    const long o = (long)j * op.ldm1 + mu;
-    if constexpr (KIND == LP_F32) {
-        f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.wlp_hi) + o);
-        out[0] = w0;
-        out[1] = w1;
-    } else {
-        const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-        unsigned hi[4], lo[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const unsigned short ha = to_e16<KIND>(w[2 * e]), hb = to_e16<KIND>(w[2 * e + 1]);
-            hi[e] = (unsigned)ha | ((unsigned)hb << 16);
-            if constexpr (V2Traits<KIND>::NSPLIT == 2) {
-                const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
-                const unsigned short lb = f32_to_bf16_bits(w[2 * e + 1] - bf16_bits_to_f32(hb));
-                lo[e] = (unsigned)la | ((unsigned)lb << 16);
-            }
+    for (int q = 0; q < Q; ++q) {
+        const long i = (long)local * WP_CH + threadIdx.x + 256 * q;
+        if (i >= nchunk) break;
+        const int j = jq[q], mu = muq[q];
+        if constexpr (KIND == LP_F16) {
+            const float f = F16_VEC / sc[q];
+            w0[q] = w0[q] * f;
+            w1[q] = w1[q] * f;
+            if (mu == 0 && j < N) op.wsc[j] = sc[q];
         }
-        *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.wlp_hi) + o) = u32x4{hi[0], hi[1], hi[2], hi[3]};
-        if constexpr (V2Traits<KIND>::NSPLIT == 2)
-            *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.wlp_lo) + o) = u32x4{lo[0], lo[1], lo[2], lo[3]};
+        const long o = (long)j * op.ldm1 + mu;
+        if constexpr (KIND == LP_F32) {
+            f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.wlp_hi) + o);
+            out[0] = w0[q];
+            out[1] = w1[q];
+        } else {
+            const float w[8] = {w0[q].x, w0[q].y, w0[q].z, w0[q].w, w1[q].x, w1[q].y, w1[q].z, w1[q].w};
+            unsigned hi[4], lo[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const unsigned short ha = to_e16<KIND>(w[2 * e]), hb = to_e16<KIND>(w[2 * e + 1]);
+                hi[e] = (unsigned)ha | ((unsigned)hb << 16);
+                if constexpr (V2Traits<KIND>::NSPLIT == 2) {
+                    const unsigned short la = f32_to_bf16_bits(w[2 * e] - bf16_bits_to_f32(ha));
+                    const unsigned short lb = f32_to_bf16_bits(w[2 * e + 1] - bf16_bits_to_f32(hb));
+                    lo[e] = (unsigned)la | ((unsigned)lb << 16);
+                }
+            }
+            *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.wlp_hi) + o) = u32x4{hi[0], hi[1], hi[2], hi[3]};
+            if constexpr (V2Traits<KIND>::NSPLIT == 2)
+                *reinterpret_cast<u32x4*>(reinterpret_cast<E*>(op.wlp_lo) + o) = u32x4{lo[0], lo[1], lo[2], lo[3]};
+        }
     }
 }
 
-template <int KIND, int BM, int BN, int D, int PM, int FW>
-__global__ __launch_bounds__(GNT) void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
+// 128 x 128 tiles keep three workgroups per CU: the register budget is 168
+// VGPRs (the swizzled-LDS variant otherwise lands at 170, two waves).
+template <int KIND, int BM, int BN, int D, int PM, int FW, int SW>
+__global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(BM * BN <= 128 * 128 && D <= 2 && FW == 0 && KIND != LP_BF16X3 ? 3 : 1)))
+void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                    int nstage, int s, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
-    __shared__ SmemV2<KIND, BM, BN> sm;
+    __shared__ SmemV2<KIND, BM, BN, SW> sm;
     LP_PICK_OP();
     const LpStage sg = ops ? st[r * nstage + s] : st1;
     const int nv = op.nv, N = nv * op.nvec;
@@ -542,11 +591,11 @@ static LpGrid lp_grid(int prec, int c1, int c3, int nmu, int nv, int nc, int nve
     const LpCfg k1 = K1_CFG[cfg_set_prec(prec)][c1], k3 = K3_CFG[cfg_set_prec(prec)][c3];
     LpGrid g;
     g.xscale = prec == PREC_F16 ? nvec * XS_SPLIT : 0;
-    g.xprep = nvec * ((nv + 31) / 32) * (ldc1 / 64);
+    g.xprep = nvec * ((nv + XP_V - 1) / XP_V) * (ldc1 / XP_C);
     g.k1 = xcd_grid((nmu + k1.bm - 1) / k1.bm, (nv * nvec + k1.bn - 1) / k1.bn, k1.pm);
     g.red = (int)(((long)(nvec + 1) * ldm1 + 255) / 256);
     g.k2 = ((nmu + L2K2 - 1) / L2K2) * ((nvec + L2K2 - 1) / L2K2);
-    g.wprep = (int)(((long)(nv * nvec + 1) * (ldm1 / 8) + 255) / 256);
+    g.wprep = (int)(((long)(nv * nvec + 1) * (ldm1 / 8) + WP_CH - 1) / WP_CH);
     g.k3 = xcd_grid((nc + k3.bm - 1) / k3.bm, (nv * nvec + k3.bn - 1) / k3.bn, k3.pm);
     return g;
 }
@@ -555,7 +604,8 @@ template <int KIND, int C>
 static void launch_k1(hipStream_t st, int nblk, const LpOp* ops, const LpOp& op1, const int* pre, int nreq)
 {
     constexpr LpCfg c = K1_CFG[cfg_set(KIND)][C];
-    hipLaunchKernelGGL((k1v2_kernel<KIND, c.bm, c.bn, c.d, c.pm>), dim3(nblk), dim3(GNT), 0, st, ops, op1, pre, nreq);
+    hipLaunchKernelGGL((k1v2_kernel<KIND, c.bm, c.bn, c.d, c.pm, c.sw>), dim3(nblk), dim3(GNT), 0, st, ops, op1, pre,
+                       nreq);
 }
 
 template <int KIND, int C>
@@ -563,8 +613,8 @@ static void launch_k3(hipStream_t st, int nblk, const LpOp* ops, const LpOp& op1
                       const LpStage& st1, int nstage, int s, const int* pre, int nreq)
 {
     constexpr LpCfg c = K3_CFG[cfg_set(KIND)][C];
-    hipLaunchKernelGGL((k3v2_kernel<KIND, c.bm, c.bn, c.d, c.pm, c.fw>), dim3(nblk), dim3(GNT), 0, st, ops, op1, stg, st1,
-                       nstage, s, pre, nreq);
+    hipLaunchKernelGGL((k3v2_kernel<KIND, c.bm, c.bn, c.d, c.pm, c.fw, c.sw>), dim3(nblk), dim3(GNT), 0, st, ops, op1,
+                       stg, st1, nstage, s, pre, nreq);
 }
 
 template <int KIND>
@@ -667,6 +717,7 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
     d.m32t = op->m32t.as<float>(); d.psiv32 = op->psiv32.as<float>();
     d.psi = op->psi.as<double>(); d.ldp = op->ld_psi;
     d.sig = op->psi_scale.as<float>();
+    d.xsig = op->xsig.as<float>() + (tda ? op->ncv : 0);
     d.sqrtr = op->sqrtr.as<double>(); d.dfull = op->diag_full.as<double>(); d.dtda = op->diag_tda.as<double>();
     d.nmu = op->n_intp; d.nv = op->nval; d.nc = op->ncond; d.nvec = nvec; d.ldc1 = op->ldc1; d.ldm1 = op->ldm1;
     d.ld = ld; d.tda = tda;

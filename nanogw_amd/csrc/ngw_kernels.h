@@ -110,6 +110,7 @@ struct LpOp {
     const float* m32t; const float* psiv32;
     const double* psi; long ldp;
     const float* sig;
+    const float* xsig;  // x' = xsig o x: sigma_c sqrtR(v,c), or sigma_c for TDA (ncv)
     const double* sqrtr; const double* dfull; const double* dtda;
     int nmu, nv, nc, nvec, ldc1, ldm1;
     long ld; int tda;
