@@ -199,6 +199,33 @@ int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, const doubl
                             double trunc, double ntrans, double* mmtrx, int ld_m);
 
 /*
+ * Same V build with zeta already on the device (d_zeta, ld_zeta), as left by
+ * ngw_isdf_zeta_dev: the psi -> zeta -> M chain stays device-resident.
+ */
+int ngw_isdf_coulomb_kernel_dev(ngw_context* ctx, int ngrid, int n_intp, const double* d_zeta,
+                                long ld_zeta, const int* grid, const int* nfft, double h,
+                                double trunc, double ntrans, double* mmtrx, int ld_m);
+
+/*
+ * ISDF interpolation vectors, replaces the zeta solve of
+ * isdf_parallel_sym_UltraLowMem (shared/isdf_parallel_sym_UltraLowMem.F90:
+ * 448-642: P, Q GEMMs, A = P o Q at the points, B = P o Q on the grid,
+ * pdgesv) on the device:
+ *   psi(ld_psi, nband)   orbitals on the ngrid points, occupied bands first
+ *   intp(n_intp)         0-based grid indices of the interpolation points
+ *   A(mu,nu) = [sum_{v<nval} psi(mu,v) psi(nu,v)] [sum_n psi(mu,n) psi(nu,n)]
+ *              (+ ridge tr(A)/n_intp on the diagonal when ridge > 0)
+ *   B(r,nu)  = [sum_{v<nval} psi(r,v) psi(nu,v)] [sum_n psi(r,n) psi(nu,n)]
+ *   zeta(ld_zeta, n_intp) = B A^-1
+ * A^-1 by Cholesky (LU when A is not numerically positive definite:
+ * *used_lu = 1).  The _dev form takes and returns device arrays.
+ */
+int ngw_isdf_zeta(ngw_context* ctx, int ngrid, int nband, int nval, const double* psi, long ld_psi,
+                  int n_intp, const int* intp, double ridge, double* zeta, long ld_zeta, int* used_lu);
+int ngw_isdf_zeta_dev(ngw_context* ctx, int ngrid, int nband, int nval, const double* d_psi, long ld_psi,
+                      int n_intp, const int* intp, double ridge, double* d_zeta, long ld_zeta, int* used_lu);
+
+/*
  * Device-pointer operator apply for callers that keep vectors on the GPU
  * (the serving layer): same as ngw_matvec_isdf_block with d_vec, d_hvec
  * device pointers of (ncv, blksz) with leading dimension ld.

@@ -43,6 +43,12 @@ _SIGS = {
     "ngw_polyfit_sqrt": ([C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp], C.c_int),
     "ngw_isdf_coulomb_kernel": ([_vp, C.c_int, C.c_int, _dp, C.c_long, _ip, _ip, C.c_double,
                                  C.c_double, C.c_double, _dp, C.c_int], C.c_int),
+    "ngw_isdf_coulomb_kernel_dev": ([_vp, C.c_int, C.c_int, _vp, C.c_long, _ip, _ip, C.c_double,
+                                     C.c_double, C.c_double, _dp, C.c_int], C.c_int),
+    "ngw_isdf_zeta": ([_vp, C.c_int, C.c_int, C.c_int, _dp, C.c_long, C.c_int, _ip, C.c_double, _dp, C.c_long,
+                       _ip], C.c_int),
+    "ngw_isdf_zeta_dev": ([_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_long, C.c_int, _ip, C.c_double, _vp,
+                           C.c_long, _ip], C.c_int),
     "ngw_matvec_isdf_block_dev": ([_vp, _vp, _vp, C.c_long, C.c_int, C.c_int], C.c_int),
     "ngw_profile_enable": ([_vp, C.c_int], C.c_int),
     "ngw_profile_reset": ([_vp], C.c_int),

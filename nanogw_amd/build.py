@@ -58,7 +58,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         raise RuntimeError("libngw build failed")
     tmp = LIB + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
+           "-L/opt/rocm/lib", "-lhipfft", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

@@ -105,9 +105,11 @@ __global__ void coulomb_kernel(int n1, int n2, int n3, double h, double trunc, d
 
 using namespace ngw;
 
-extern "C" int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, const double* zeta,
-                                       long ld_zeta, const int* grid, const int* nfft, double h,
-                                       double trunc, double ntrans, double* mmtrx, int ld_m)
+// zeta_dev: zeta is a device array (ld_zeta) used in place; otherwise a host
+// array copied to the device.  M is returned to the host.
+static int coulomb_impl(ngw_context* ctx, int ngrid, int n_intp, const double* zeta, long ld_zeta, bool zeta_dev,
+                        const int* grid, const int* nfft, double h, double trunc, double ntrans, double* mmtrx,
+                        int ld_m)
 {
     if (!ctx) return fail(NGW_ERR_ARG, "null context");
     if (ngrid <= 0 || n_intp <= 0 || ld_zeta < ngrid || ld_m < n_intp || !zeta || !grid || !nfft || !mmtrx)
@@ -133,7 +135,7 @@ extern "C" int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, 
         DevBuf* b[8];
         ~Guard() { for (DevBuf* x : b) x->release(); }
     } guard{{&dzeta, &dvz, &dbox, &dfbox, &dvc, &doff, &dgrid, &dm}};
-    NGW_TRY(dzeta.ensure(sizeof(double) * (size_t)ngrid * n_intp));
+    if (!zeta_dev) NGW_TRY(dzeta.ensure(sizeof(double) * (size_t)ngrid * n_intp));
     NGW_TRY(dvz.ensure(sizeof(double) * (size_t)ngrid * nb));
     NGW_TRY(dbox.ensure(sizeof(double) * (size_t)boxsz * nb));
     NGW_TRY(dfbox.ensure(sizeof(double) * 2 * (size_t)csz * nb));
@@ -141,11 +143,18 @@ extern "C" int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, 
     NGW_TRY(doff.ensure(sizeof(long) * (size_t)ngrid));
     NGW_TRY(dgrid.ensure(sizeof(int) * 3 * (size_t)ngrid));
     NGW_TRY(dm.ensure(sizeof(double) * (size_t)n_intp * n_intp));
-    if (ld_zeta == ngrid) {
-        NGW_HIP(hipMemcpyAsync(dzeta.ptr, zeta, sizeof(double) * (size_t)ngrid * n_intp, hipMemcpyHostToDevice, s));
-    } else {
-        NGW_HIP(hipMemcpy2DAsync(dzeta.ptr, sizeof(double) * ngrid, zeta, sizeof(double) * ld_zeta,
-                                 sizeof(double) * ngrid, n_intp, hipMemcpyHostToDevice, s));
+    const double* zd = zeta;  // device zeta and its leading dimension
+    long ldz = ld_zeta;
+    if (!zeta_dev) {
+        if (ld_zeta == ngrid) {
+            NGW_HIP(hipMemcpyAsync(dzeta.ptr, zeta, sizeof(double) * (size_t)ngrid * n_intp, hipMemcpyHostToDevice,
+                                   s));
+        } else {
+            NGW_HIP(hipMemcpy2DAsync(dzeta.ptr, sizeof(double) * ngrid, zeta, sizeof(double) * ld_zeta,
+                                     sizeof(double) * ngrid, n_intp, hipMemcpyHostToDevice, s));
+        }
+        zd = dzeta.as<double>();
+        ldz = ngrid;
     }
     NGW_HIP(hipMemcpyAsync(dgrid.ptr, grid, sizeof(int) * 3 * (size_t)ngrid, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(box_offsets_kernel, dim3((ngrid + 255) / 256), dim3(256), 0, s, dgrid.as<int>(), ngrid,
@@ -170,7 +179,7 @@ extern "C" int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, 
         const int cb = std::min(nb, n_intp - c0);
         NGW_HIP(hipMemsetAsync(dbox.ptr, 0, sizeof(double) * (size_t)boxsz * nb, s));
         hipLaunchKernelGGL(scatter_kernel, dim3(std::min((ngrid + 255) / 256, 1024), cb), dim3(256), 0, s,
-                           dzeta.as<double>() + (size_t)c0 * ngrid, (long)ngrid, ngrid, doff.as<long>(), boxsz,
+                           zd + (size_t)c0 * ldz, ldz, ngrid, doff.as<long>(), boxsz,
                            dbox.as<double>(), cb);
         NGW_HIP(hipGetLastError());
         NGW_FFT(hipfftExecD2Z(pf, dbox.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(dfbox.ptr)));
@@ -182,7 +191,7 @@ extern "C" int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, 
                            dbox.as<double>(), boxsz, doff.as<long>(), ngrid, scale, dvz.as<double>(), (long)ngrid);
         NGW_HIP(hipGetLastError());
         // M(:, c0:c0+cb) = norm * zeta^T vz
-        NGW_TRY(gemm_f64_tn(s, n_intp, cb, ngrid, norm, dzeta.as<double>(), ngrid, dvz.as<double>(), ngrid,
+        NGW_TRY(gemm_f64_tn(s, n_intp, cb, ngrid, norm, zd, ldz, dvz.as<double>(), ngrid,
                             dm.as<double>() + (size_t)c0 * n_intp, n_intp));
     }
     std::vector<double> hm((size_t)n_intp * n_intp);
@@ -191,4 +200,18 @@ extern "C" int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, 
     for (int j = 0; j < n_intp; ++j)
         std::memcpy(mmtrx + (size_t)j * ld_m, &hm[(size_t)j * n_intp], sizeof(double) * n_intp);
     return NGW_OK;
+}
+
+extern "C" int ngw_isdf_coulomb_kernel(ngw_context* ctx, int ngrid, int n_intp, const double* zeta,
+                                       long ld_zeta, const int* grid, const int* nfft, double h,
+                                       double trunc, double ntrans, double* mmtrx, int ld_m)
+{
+    return coulomb_impl(ctx, ngrid, n_intp, zeta, ld_zeta, false, grid, nfft, h, trunc, ntrans, mmtrx, ld_m);
+}
+
+extern "C" int ngw_isdf_coulomb_kernel_dev(ngw_context* ctx, int ngrid, int n_intp, const double* d_zeta,
+                                           long ld_zeta, const int* grid, const int* nfft, double h,
+                                           double trunc, double ntrans, double* mmtrx, int ld_m)
+{
+    return coulomb_impl(ctx, ngrid, n_intp, d_zeta, ld_zeta, true, grid, nfft, h, trunc, ntrans, mmtrx, ld_m);
 }

@@ -145,6 +145,19 @@ module ngw_c
       real(c_double), intent(out) :: mmtrx(*)
     end function ngw_isdf_coulomb_kernel
 
+    integer(c_int) function ngw_isdf_zeta(ctx, ngrid, nband, nval, psi, ld_psi, n_intp, intp, ridge, &
+                                          zeta, ld_zeta, used_lu) bind(C, name='ngw_isdf_zeta')
+      import :: c_ptr, c_int, c_long, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngrid, nband, nval, n_intp
+      integer(c_long), value :: ld_psi, ld_zeta
+      real(c_double), intent(in) :: psi(*)
+      integer(c_int), intent(in) :: intp(*)
+      real(c_double), value :: ridge
+      real(c_double), intent(out) :: zeta(*)
+      integer(c_int), intent(out) :: used_lu
+    end function ngw_isdf_zeta
+
   end interface
 
 contains

@@ -198,6 +198,43 @@ class IsdfContext:
             n_intp), "ngw_isdf_coulomb_kernel")
         return out
 
+    def isdf_coulomb_kernel_dev(self, d_zeta: int, ld_zeta: int, ngrid: int, n_intp: int, gidx, nfft, h, trunc,
+                                ntrans=1.0):
+        """Same V build on a device zeta (ld_zeta, n_intp), e.g. from isdf_zeta_dev."""
+        grid = np.ascontiguousarray(np.asarray(gidx, np.int32).reshape(ngrid, 3))
+        nf = np.ascontiguousarray(np.asarray(nfft, np.int32).reshape(3))
+        out = np.zeros((n_intp, n_intp), order="F")
+        _lib.check(self._lib.ngw_isdf_coulomb_kernel_dev(
+            self._h, int(ngrid), int(n_intp), int(d_zeta), int(ld_zeta), grid.ctypes.data_as(C.POINTER(C.c_int)),
+            nf.ctypes.data_as(C.POINTER(C.c_int)), float(h), float(trunc), float(ntrans), _p(out),
+            n_intp), "ngw_isdf_coulomb_kernel_dev")
+        return out
+
+    def isdf_zeta(self, psi, intp, nv, ridge=0.0):
+        """zeta (ngrid, n_intp) = B A^-1 on the device from host orbitals psi
+        (ngrid, nband) and 0-based point indices (isdf_parallel_sym_UltraLowMem.F90:448-642).
+        Returns (zeta, used_lu)."""
+        psi = _f64(psi)
+        ngrid, nband = psi.shape
+        ii = np.ascontiguousarray(np.asarray(intp, np.int32))
+        zeta = np.zeros((ngrid, len(ii)), order="F")
+        lu = C.c_int(0)
+        _lib.check(self._lib.ngw_isdf_zeta(
+            self._h, ngrid, nband, int(nv), _p(psi), ngrid, len(ii), ii.ctypes.data_as(C.POINTER(C.c_int)),
+            float(ridge), _p(zeta), ngrid, C.byref(lu)), "ngw_isdf_zeta")
+        return zeta, bool(lu.value)
+
+    def isdf_zeta_dev(self, d_psi: int, ld_psi: int, ngrid: int, nband: int, nv: int, intp, ridge, d_zeta: int,
+                      ld_zeta: int):
+        """Device form of isdf_zeta; returns whether the LU fallback was used."""
+        ii = np.ascontiguousarray(np.asarray(intp, np.int32))
+        lu = C.c_int(0)
+        _lib.check(self._lib.ngw_isdf_zeta_dev(
+            self._h, int(ngrid), int(nband), int(nv), int(d_psi), int(ld_psi), len(ii),
+            ii.ctypes.data_as(C.POINTER(C.c_int)), float(ridge), int(d_zeta), int(ld_zeta), C.byref(lu)),
+            "ngw_isdf_zeta_dev")
+        return bool(lu.value)
+
     # ------------------------------------------------------------ device pointers
     def matvec_dev(self, d_vec: int, d_hvec: int, ld: int, blksz: int, tamm_d=False):
         _lib.check(self._lib.ngw_matvec_isdf_block_dev(self._h, d_vec, d_hvec, int(ld), int(blksz),
