@@ -226,6 +226,19 @@ int ngw_isdf_zeta_dev(ngw_context* ctx, int ngrid, int nband, int nval, const do
                       int n_intp, const int* intp, double ridge, double* d_zeta, long ld_zeta, int* used_lu);
 
 /*
+ * ISDF interpolation points by rho-weighted centroidal Voronoi tessellation,
+ * replaces cvt (shared/cvt.F90:8-195): Lloyd iterations of nearest-centroid
+ * assignment and rho-weighted centroids (on the GPU), each centroid snapped
+ * in index order to the nearest grid point not already taken, until the
+ * summed displacement is below tol (bohr) or max_iter iterations.
+ *   grid(3, ngrid)    integer lattice coordinates (bohr = h x coordinate)
+ *   rho(ngrid)        weights; init(3, n_intp) starting centroids (bohr)
+ *   intp(n_intp)      output: 0-based grid indices; *iters iterations run
+ */
+int ngw_cvt_points(ngw_context* ctx, int ngrid, const int* grid, const double* rho, int n_intp,
+                   const double* init, double h, double tol, int max_iter, int* intp, int* iters);
+
+/*
  * Device-pointer operator apply for callers that keep vectors on the GPU
  * (the serving layer): same as ngw_matvec_isdf_block with d_vec, d_hvec
  * device pointers of (ncv, blksz) with leading dimension ld.

@@ -49,6 +49,8 @@ _SIGS = {
                        _ip], C.c_int),
     "ngw_isdf_zeta_dev": ([_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_long, C.c_int, _ip, C.c_double, _vp,
                            C.c_long, _ip], C.c_int),
+    "ngw_cvt_points": ([_vp, C.c_int, _ip, _dp, C.c_int, _dp, C.c_double, C.c_double, C.c_int, _ip, _ip],
+                       C.c_int),
     "ngw_matvec_isdf_block_dev": ([_vp, _vp, _vp, C.c_long, C.c_int, C.c_int], C.c_int),
     "ngw_profile_enable": ([_vp, C.c_int], C.c_int),
     "ngw_profile_reset": ([_vp], C.c_int),

@@ -210,6 +210,21 @@ class IsdfContext:
             n_intp), "ngw_isdf_coulomb_kernel_dev")
         return out
 
+    def cvt_points(self, gidx, rho, init, h, tol=1e-3, max_iter=300):
+        """ISDF points by rho-weighted CVT (shared/cvt.F90:8-195) on the GPU.
+        gidx (ngrid, 3) lattice coordinates, rho (ngrid), init (n_intp, 3) bohr.
+        Returns (0-based grid indices, iterations run)."""
+        grid = np.ascontiguousarray(np.asarray(gidx, np.int32).reshape(-1, 3))
+        r = np.ascontiguousarray(np.asarray(rho, np.float64))
+        ini = np.ascontiguousarray(np.asarray(init, np.float64).reshape(-1, 3))
+        out = np.zeros(ini.shape[0], np.int32)
+        it = C.c_int(0)
+        _lib.check(self._lib.ngw_cvt_points(
+            self._h, grid.shape[0], grid.ctypes.data_as(C.POINTER(C.c_int)), _p(r), ini.shape[0], _p(ini),
+            float(h), float(tol), int(max_iter), out.ctypes.data_as(C.POINTER(C.c_int)), C.byref(it)),
+            "ngw_cvt_points")
+        return out.astype(np.int64), it.value
+
     def isdf_zeta(self, psi, intp, nv, ridge=0.0):
         """zeta (ngrid, n_intp) = B A^-1 on the device from host orbitals psi
         (ngrid, nband) and 0-based point indices (isdf_parallel_sym_UltraLowMem.F90:448-642).

@@ -20,6 +20,10 @@ Pinning (tests/test_oracle_*.py):
   * spline/Newton: exactness on cubic tables, one-step Newton on linear f;
   * polyfit: numpy.linalg.lstsq on well-conditioned cases.  The npoly = 12
     LAPACK dgesvx coefficients are parity unpinned; they are pinned inputs.
+  * cvt_lattice: a brute-force Lloyd step (full distance matrix, weighted
+    means, nearest free grid point by a scan of all points).  The point set
+    of the reference's own CVT depends on gfortran's random_number and is
+    parity unpinned; points are shared inputs.
 """
 from __future__ import annotations
 
@@ -284,3 +288,81 @@ def isdf_kernel(zeta, gidx, nfft, h, trunc, ntrans=1):
     """M = (ntrans/h^3) zeta^T (V zeta), isdf_parallel_sym_UltraLowMem.F90:312, 749, 865-867."""
     vz = poisson(zeta, gidx, nfft, h, trunc)
     return (ntrans / h ** 3) * (zeta.T @ vz)
+
+
+def _snap_one(table, lo, taken, c):
+    """Nearest free grid point to centroid c (lattice units): cubic shells
+    s = 0, 1, ... around the rounded centroid, each scanned in (z, y, x)
+    order, first point at the smallest distance kept (cvt.F90:104-128 picks
+    the nearest unused grid point; this fixes the order among ties)."""
+    r = [int(np.floor(v + 0.5)) if v >= 0 else -int(np.floor(-v + 0.5)) for v in c]  # C lround
+    n = table.shape[::-1]
+    best, bd = -1, 1e300
+    s = 0
+    while True:
+        for dk in range(-s, s + 1):
+            for dj in range(-s, s + 1):
+                for di in range(-s, s + 1):
+                    if max(abs(di), abs(dj), abs(dk)) != s:
+                        continue
+                    i, j, k = r[0] + di - lo[0], r[1] + dj - lo[1], r[2] + dk - lo[2]
+                    if i < 0 or j < 0 or k < 0 or i >= n[0] or j >= n[1] or k >= n[2]:
+                        continue
+                    g = int(table[k, j, i])
+                    if g < 0 or taken[g]:
+                        continue
+                    ex, ey, ez = (r[0] + di) - c[0], (r[1] + dj) - c[1], (r[2] + dk) - c[2]
+                    d = (ex * ex + ey * ey) + ez * ez
+                    if d < bd:
+                        bd, best = d, g
+        if best >= 0 and s + 0.5 > np.sqrt(bd):
+            return best
+        if s > 4 * sum(n):
+            return best
+        s += 1
+
+
+def cvt_lattice(gidx, rho, init, h, tol=1e-3, max_iter=300):
+    """rho-weighted Lloyd iteration of shared/cvt.F90:8-195 in lattice units:
+    nearest-centroid assignment (:77-88; lowest index on ties), weighted
+    centroids with the old position kept below weight 1e-10 (:92-103), snap
+    in index order to the nearest free grid point (:104-128), stop when the
+    summed displacement (bohr) is below tol (:131-135).  Small grids only
+    (dense distance blocks, Python snap loop).  Returns (intp, iterations)."""
+    g = np.asarray(gidx, np.int64)
+    gf = g.astype(np.float64)
+    rho = np.asarray(rho, np.float64)
+    lo = g.min(axis=0)
+    n = g.max(axis=0) - lo + 1
+    table = -np.ones((n[2], n[1], n[0]), np.int64)
+    table[g[:, 2] - lo[2], g[:, 1] - lo[1], g[:, 0] - lo[0]] = np.arange(len(g))
+    cen = np.asarray(init, np.float64) / h
+    old = cen.copy()
+    nc = len(cen)
+    intp = np.zeros(nc, np.int64)
+    it = 0
+    for it in range(1, max_iter + 1):
+        owner = np.empty(len(g), np.int64)
+        for r0 in range(0, len(g), 4096):
+            p = gf[r0:r0 + 4096]
+            dx = p[:, None, 0] - cen[None, :, 0]
+            dy = p[:, None, 1] - cen[None, :, 1]
+            dz = p[:, None, 2] - cen[None, :, 2]
+            owner[r0:r0 + 4096] = np.argmin((dx * dx + dy * dy) + dz * dz, axis=1)
+        w = np.bincount(owner, weights=rho, minlength=nc)
+        wx = [np.bincount(owner, weights=rho * gf[:, k], minlength=nc) for k in range(3)]
+        taken = np.zeros(len(g), bool)
+        diff = 0.0
+        new = np.empty_like(cen)
+        for c in range(nc):
+            cand = [wx[k][c] / w[c] for k in range(3)] if w[c] >= 1e-10 else list(old[c])
+            gi = _snap_one(table, lo, taken, cand)
+            taken[gi] = True
+            intp[c] = gi
+            new[c] = gf[gi]
+            diff += np.sqrt(((new[c] - old[c]) ** 2).sum()) * h
+        old = new.copy()
+        cen = new
+        if diff < tol:
+            break
+    return intp, it

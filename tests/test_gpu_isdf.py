@@ -83,3 +83,25 @@ def test_zeta_argument_checks(ngw_ctx, small_fit):
         ngw_ctx.isdf_zeta(m.psi, bad, m.nv)
     with pytest.raises(NgwError):
         ngw_ctx.isdf_zeta(m.psi, m.intp, m.nv, ridge=-1.0)
+
+
+def test_cvt_points_vs_oracle(ngw_ctx):
+    """Device CVT against the numpy restatement with the same tie rules:
+    identical point sets and iteration counts; points unique and on the grid."""
+    from nanogw_amd import synth
+    from oracle import oracle as O
+
+    for (radius, h, nv, n_intp, seed) in [(4.0, 0.5, 3, 24, 5), (5.0, 0.5, 4, 60, 6)]:
+        gidx = synth.sphere_grid(radius, h)
+        rng = np.random.default_rng(seed)
+        psi = synth.orbitals(gidx, h, nv + 6, rng, radius)
+        rho = 2.0 * np.sum(psi[:, :nv] ** 2, axis=1)
+        pts = gidx.astype(np.float64) * h
+        lo, hi = pts.min(axis=0), pts.max(axis=0)
+        init = lo[None, :] + rng.uniform(size=(n_intp, 3)) * 0.9 * (hi - lo)[None, :]
+        got, it = ngw_ctx.cvt_points(gidx, rho, init, h, tol=1e-3, max_iter=100)
+        ref, it_ref = O.cvt_lattice(gidx, rho, init, h, tol=1e-3, max_iter=100)
+        print("cvt", radius, n_intp, "iterations", it, it_ref, "mismatched points", int(np.sum(got != ref)))
+        assert len(set(got.tolist())) == n_intp and got.min() >= 0 and got.max() < len(gidx)
+        assert it == it_ref
+        assert np.array_equal(got, ref)

@@ -233,3 +233,39 @@ def test_isdf_kernel_symmetric_psd(O, tiny):
     M = O.isdf_kernel(tiny.zeta, tiny.gidx, tiny.nfft, tiny.h, tiny.trunc)
     assert rel_err(M, M.T) < 1e-10
     assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > -1e-8 * np.abs(M).max()
+
+
+def test_cvt_lattice_lloyd_step_by_brute_force(O):
+    """cvt_lattice (the CVT restatement the device CVT is checked against),
+    pinned by a brute-force Lloyd step: from the restatement's points after
+    k - 1 iterations, nearest-centroid assignment over the full distance
+    matrix, rho-weighted means, and for each centroid in order the nearest
+    grid point not yet taken (by a scan of every grid point) give the
+    restatement's points after k iterations."""
+    from nanogw_amd import synth
+
+    radius, h, nv, n_intp = 3.5, 0.5, 3, 20
+    gidx = synth.sphere_grid(radius, h)
+    rng = np.random.default_rng(9)
+    psi = synth.orbitals(gidx, h, nv + 4, rng, radius)
+    rho = 2.0 * np.sum(psi[:, :nv] ** 2, axis=1)
+    pts = gidx.astype(np.float64)
+    lo, hi = pts.min(axis=0) * h, pts.max(axis=0) * h
+    init = lo[None, :] + rng.uniform(size=(n_intp, 3)) * 0.9 * (hi - lo)[None, :]
+    for k in (1, 2, 3):
+        prev, _ = O.cvt_lattice(gidx, rho, init, h, tol=-1.0, max_iter=k - 1) if k > 1 else (None, 0)
+        cen = init / h if prev is None else pts[prev]
+        got, _ = O.cvt_lattice(gidx, rho, init, h, tol=-1.0, max_iter=k)
+        d = ((pts[:, None, :] - cen[None, :, :]) ** 2).sum(axis=2)
+        owner = np.argmin(d, axis=1)
+        taken = np.zeros(len(pts), bool)
+        for c in range(n_intp):
+            mine = owner == c
+            w = rho[mine].sum()
+            cand = (rho[mine, None] * pts[mine]).sum(axis=0) / w if w >= 1e-10 else cen[c]
+            dist = ((pts - cand) ** 2).sum(axis=1)
+            dist[taken] = np.inf
+            best = dist.min()
+            assert abs(dist[got[c]] - best) <= 1e-9 * max(1.0, best), (k, c)
+            taken[got[c]] = True
+        assert len(set(got.tolist())) == n_intp
