@@ -1,0 +1,79 @@
+"""nanogw.in reading (ESDF rules of shared/esdf_module.f90) and the
+printsigma table / hmat_qp_nostatic writers (SIGMA/printsigma.F90:167-188)."""
+import numpy as np
+import pytest
+
+from nanogw_amd import nanogw_files as F
+
+SAMPLE = """
+# ISDF Lanczos GW run
+Lanczos_GW
+doisdf                      ! ISDF on
+lessmemory = 3
+num_isdf_points: 480
+intp_type 1
+lanczos_niter   200
+LANCZOS_NPOLY   10          ; labels are case-insensitive
+jnblock 64
+energy_range 54.4228 eV
+energy_data_points 200
+dynamic_energy_resolution 0.01 hartree
+begin diag
+  range 7 9
+  10
+end diag
+lanczos_niter 15            # second definition: the first one is used
+"""
+
+
+def test_read_nanogw_in_keys_units_blocks():
+    s = F.read_nanogw_in(SAMPLE, is_text=True)
+    assert s.lanczos_gw and s.doisdf and not s.tamm_dancoff
+    assert s.lessmemory == 3 and s.num_isdf_points == 480 and s.intp_type == 1
+    assert s.lanczos_niter == 200 and s.lanczos_npoly == 10 and s.jnblock == 64
+    assert s.nen == 200
+    assert s.deltae == pytest.approx(54.4228 / F.RYD, rel=1e-12)
+    # 0.01 Ha in Ry with the ESDF unit table (4.35982e-18 J / 2.17991e-18 J)
+    assert s.ecuts == pytest.approx(0.01 * 4.35982e-18 / 2.17991e-18, rel=1e-14)
+    assert s.diag == [7, 8, 9, 10]
+    d = F.read_nanogw_in("", is_text=True)
+    assert (d.lanczos_niter, d.lanczos_npoly, d.jnblock, d.nen) == (9, 12, 400, 21)
+    assert d.deltae == pytest.approx(20.0 / F.RYD) and d.ecuts == 0.005 and d.diag == []
+
+
+def test_esdf_label_reduction_and_errors():
+    assert F.esdf_reduce(" Lanczos-GW. ") == "lanczosgw"
+    assert F.esdf_reduce("Lanczos_GW") == "lanczos_gw"
+    with pytest.raises(ValueError):
+        F.read_nanogw_in("begin diag\n 3\n", is_text=True)
+    with pytest.raises(ValueError):
+        F.read_nanogw_in("energy_range 3 furlongs\n", is_text=True)
+
+
+def test_sigma_row_and_hmat_formats():
+    # (i6,f8.3,11(1x,f11.5)): 6 + 8 + 11 x 12 characters
+    row = F.sigma_row(100, 2.0, -12.448464, 0.0, -14.1, 0.0, 1.2345678, 0.0, -12.8654322, 0.87654, -0.0123,
+                      -12.448464, -12.45)
+    assert len(row) == 6 + 8 + 11 * 12
+    assert row[:14] == "   100   2.000"
+    assert row[14:26] == "   -12.44846"
+    assert row.endswith("   -12.45000")
+    # (2i8,f10.3,2i4,2f10.3)
+    h = F.hmat_row(100, -12.45, 1, 1, -12.448464, -12.448464)
+    assert h == "     100     100   -12.450   1   1   -12.448   -12.448"
+    # a value that does not fit its field prints as asterisks, like Fortran
+    assert F.sigma_row(1, 0.0, 1e9, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)[14:26] == " " + "*" * 11
+    assert F.sigma_file_name(1, 0) == "sigma_nostatic_001_0000"
+    assert F.sigma_file_name(2, 13, "up") == "sigma_nostatic_up_002_0013"
+
+
+def test_qp_rows_follow_printsigma(tmp_path):
+    newton = {"eqp": np.array([-16.76, -12.45]), "zfac": np.array([0.9, 0.8]),
+              "sigma0": np.array([-15.0 + 0.1j, -13.0 + 0.0j]), "imsig": np.array([0.01, 0.02])}
+    rows = F.qp_rows([99, 100], [2.0, 0.0], [-16.0, -12.0], [-10.0, -9.0], [-14.0, -11.0], newton)
+    assert rows[0]["n"] == 100 and rows[1]["n"] == 101
+    assert rows[0]["sigc"] == pytest.approx(-1.0) and rows[0]["sig_m_vxc"] == pytest.approx(-5.0)
+    path = tmp_path / F.sigma_file_name(1, 0)
+    F.write_sigma_table(path, rows)
+    lines = path.read_text().splitlines()
+    assert lines[:4] == F.SIGMA_HEADER and len(lines) == 6
