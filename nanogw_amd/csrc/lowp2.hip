@@ -243,14 +243,21 @@ int lp2_prepare(ngw_context* ctx)
 // of each vector (XS_SPLIT workgroups per vector, eight loads in flight per
 // thread); xprep turns them into the vector's scale xsc(b).
 constexpr int XS_SPLIT = 8;
+// workgroups per vector: about one per 16384 elements, at most XS_SPLIT
+__host__ __device__ inline int xs_split(long ncv)
+{
+    const long k = (ncv + 16383) / 16384;
+    return k < 1 ? 1 : (k > XS_SPLIT ? XS_SPLIT : (int)k);
+}
 __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                      int nstage, int s, const int* pre, int nreq)
 {
     LP_PICK_OP();
-    const int b = local / XS_SPLIT, part = local - b * XS_SPLIT;
-    const double* xb = (ops ? st[r * nstage + s].x : st1.x) + (long)b * op.ld;
     const long ncv = (long)op.nv * op.nc;
-    const long chunk = (ncv + XS_SPLIT - 1) / XS_SPLIT;
+    const int split = xs_split(ncv);
+    const int b = local / split, part = local - b * split;
+    const double* xb = (ops ? st[r * nstage + s].x : st1.x) + (long)b * op.ld;
+    const long chunk = (ncv + split - 1) / split;
     const long lo = part * chunk, hi = min(ncv, lo + chunk);
     double m = 0.0;
     for (long base = lo + threadIdx.x; base < hi; base += 2048) {
@@ -264,28 +271,40 @@ __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, 
         for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
     }
     const float mb = block_max256((float)m);
-    if (threadIdx.x == 0) op.xpart[local] = mb;
+    if (threadIdx.x == 0) op.xpart[b * XS_SPLIT + part] = mb;
 }
 
 // x'(j = v + nv b, c) = xsig(v,c) x_b(v,c), xsig = sigma_c s(v,c) (sigma_c
-// for TDA), per workgroup one vector's (64 c) x (up to 128 v) block: the reads
-// are one contiguous run of x (all of v for 64 consecutive c), the block is
-// transposed through LDS, and each thread writes eight consecutive c in one
-// 16- or 32-byte store.
-constexpr int XP_V = 128, XP_C = 64;
+// for TDA).  A workgroup converts one vector's block of VT = min(nv, 128)
+// consecutive v by CT = 64 k consecutive c, k the largest power of two whose
+// LDS tile CT (VT + 1) fits XP_TILE floats (and CT <= ldc1): 4-8k elements
+// whatever nv is.  For
+// nv <= 128 the block is one contiguous run of x, read flat and fully
+// coalesced; it is transposed through LDS and each thread writes eight
+// consecutive c in one 16- or 32-byte store.
+constexpr int XP_V = 128, XP_TILE = 10240;
+__host__ __device__ inline int xp_vt(int nv) { return nv < XP_V ? nv : XP_V; }
+__host__ __device__ inline int xp_ct(int nv, int ldc1)
+{
+    int k = 1;
+    while (2 * k * 64 * (xp_vt(nv) + 1) <= XP_TILE && 2 * k * 64 <= ldc1) k *= 2;
+    return 64 * k;
+}
 template <int KIND>
 __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                     int nstage, int s, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
-    __shared__ float tile[XP_C][XP_V + 1];
+    __shared__ float tile[XP_TILE];  // [c][v], pitch vt + 1
     LP_PICK_OP();
     const double* x = ops ? st[r * nstage + s].x : st1.x;
-    const int nvt = (op.nv + XP_V - 1) / XP_V, nct = op.ldc1 / XP_C;
+    const int vt = xp_vt(op.nv), ct = xp_ct(op.nv, op.ldc1);
+    const int nvt = (op.nv + vt - 1) / vt, nct = (op.ldc1 + ct - 1) / ct;
     const int b = local / (nvt * nct);
     const int rem = local - b * nvt * nct;
-    const int v0 = (rem / nct) * XP_V, c0 = (rem % nct) * XP_C;
-    const int vn = min(XP_V, op.nv - v0), cn = max(0, min(XP_C, op.nc - c0));
+    const int v0 = (rem / nct) * vt, c0 = (rem % nct) * ct;
+    const int vn = min(vt, op.nv - v0), cn = max(0, min(ct, op.nc - c0));
+    const int pitch = vt + 1;
     if (local == 0)  // the zero row after the last x' row
         for (int c = threadIdx.x; c < op.ldc1; c += 256)
             put_v2<KIND>((E*)op.xlp_hi, (E*)op.xlp_lo, (long)op.nv * op.nvec * op.ldc1 + c, 0.0f);
@@ -293,39 +312,47 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
     double f16s = 1.0;
     if constexpr (KIND == LP_F16) {
         float m = 0.0f;
-#pragma unroll
-        for (int k = 0; k < XS_SPLIT; ++k) m = fmaxf(m, op.xpart[b * XS_SPLIT + k]);
+        const int split = xs_split((long)op.nv * op.nc);
+        for (int k = 0; k < split; ++k) m = fmaxf(m, op.xpart[b * XS_SPLIT + k]);
         const float xsc = pow2_ceil(m);
         f16s = (double)F16_VEC / (double)xsc;
         if (rem == 0 && threadIdx.x == 0) op.xsc[b] = xsc;  // for K1's epilogue
     }
-    // lanes along v (128 per row, rows of one vector are nv apart in x), two
-    // rows per pass, eight rows in flight per thread
-    const int tx = threadIdx.x & (XP_V - 1), ty = threadIdx.x / XP_V;
-    const long base = v0 + tx + (long)c0 * op.nv;
-    for (int cb = 0; cb < cn; cb += 16) {
+    // flat walk e = v + vn c over the block; (c, v) from a float reciprocal
+    // with an exact correction
+    const int tot = vn * cn;
+    const long base = v0 + (long)c0 * op.nv;
+    const float inv = 1.0f / (float)vn;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += 256 * 8) {
         double xv[8], sv[8];
+        int cc[8], vv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int c = cb + ty + 2 * u;
-            const bool in = tx < vn && c < cn;
-            const long idx = base + (long)c * op.nv;
-            xv[u] = in ? xb[idx] : 0.0;
-            sv[u] = in ? (double)op.xsig[idx] : 0.0;
+            const int e = e0 + 256 * u;
+            int c = (int)((float)e * inv);
+            if (c * vn > e) --c;
+            if ((c + 1) * vn <= e) ++c;
+            cc[u] = c;
+            vv[u] = e - c * vn;
+            const long idx = base + vv[u] + (long)c * op.nv;
+            xv[u] = e < tot ? xb[idx] : 0.0;
+            sv[u] = e < tot ? (double)op.xsig[idx] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int c = cb + ty + 2 * u;
-            if (c < XP_C) tile[c][tx] = (float)(xv[u] * (sv[u] * f16s));
-        }
+        for (int u = 0; u < 8; ++u)
+            if (e0 + 256 * u < tot) tile[cc[u] * pitch + vv[u]] = (float)(xv[u] * (sv[u] * f16s));
     }
-    for (int c = ((cn + 15) / 16) * 16 + ty; c < XP_C; c += 2) tile[c][tx] = 0.0f;  // k padding past the last pass
+    for (int i = cn * pitch + threadIdx.x; i < ct * pitch; i += 256) tile[i] = 0.0f;  // k padding rows
     __syncthreads();
-    for (int t = threadIdx.x; t < vn * 8; t += 256) {
-        const int vl = t >> 3, cq = (t & 7) * 8;
+    const int cpr = ct / 8;  // 8-wide chunks per row (a power of two)
+    const int sh = __ffs(cpr) - 1;
+    const int lim = min(ct, op.ldc1 - c0);
+    for (int t = threadIdx.x; t < vn * cpr; t += 256) {
+        const int vl = t >> sh, cq = (t & (cpr - 1)) * 8;
+        if (cq >= lim) continue;
         float w[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) w[e] = tile[cq + e][vl];
+        for (int e = 0; e < 8; ++e) w[e] = tile[(cq + e) * pitch + vl];
         const long o = (long)(v0 + vl + op.nv * b) * op.ldc1 + c0 + cq;
         if constexpr (KIND == LP_F32) {
             f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.xlp_hi) + o);
@@ -590,8 +617,8 @@ static LpGrid lp_grid(int prec, int c1, int c3, int nmu, int nv, int nc, int nve
 {
     const LpCfg k1 = K1_CFG[cfg_set_prec(prec)][c1], k3 = K3_CFG[cfg_set_prec(prec)][c3];
     LpGrid g;
-    g.xscale = prec == PREC_F16 ? nvec * XS_SPLIT : 0;
-    g.xprep = nvec * ((nv + XP_V - 1) / XP_V) * (ldc1 / XP_C);
+    g.xscale = prec == PREC_F16 ? nvec * xs_split((long)nv * nc) : 0;
+    g.xprep = nvec * ((nv + xp_vt(nv) - 1) / xp_vt(nv)) * ((ldc1 + xp_ct(nv, ldc1) - 1) / xp_ct(nv, ldc1));
     g.k1 = xcd_grid((nmu + k1.bm - 1) / k1.bm, (nv * nvec + k1.bn - 1) / k1.bn, k1.pm);
     g.red = (int)(((long)(nvec + 1) * ldm1 + 255) / 256);
     g.k2 = ((nmu + L2K2 - 1) / L2K2) * ((nvec + L2K2 - 1) / L2K2);

@@ -202,3 +202,30 @@ def test_sigma_qp_mixed_vs_fp64(ngw_ctx, wellcond, prec, qp_tol_ev):
     dq = np.abs(got - exp).max()
     print(f"{prec}: max |dE_qp| = {dq * 1e3:.4f} meV, Sigma_c rel err {rel_err(sig, ref):.2e}")
     assert dq < qp_tol_ev
+
+
+@pytest.mark.parametrize("nv,nc", [(130, 20), (1, 70), (17, 200)])
+@pytest.mark.parametrize("prec,tol", [("f16", 5e-3), ("bf16x3", 1e-4)])
+def test_conversion_tiles_odd_shapes(ngw_ctx, nv, nc, prec, tol):
+    """x' conversion blocks for nv above one 128-wide tile (two v blocks, the
+    second ragged), for nv = 1 (the widest c block) and for a ragged c tail,
+    on a random well-conditioned operator against the oracle."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(nv * 1000 + nc)
+    nmu = 72
+    psi = np.asfortranarray(rng.normal(size=(nmu, nv + nc)) / np.sqrt(nmu))
+    g = rng.normal(size=(nmu, nmu)) / nmu
+    mm = np.asfortranarray(g @ g.T + np.eye(nmu))
+    e1 = np.concatenate([np.sort(rng.uniform(-1.5, -0.5, nv)), np.sort(rng.uniform(-0.2, 3.0, nc))])
+    ngw_ctx.set_isdf(psi, mm, e1, nv)
+    ngw_ctx.set_precision(prec)
+    try:
+        x = rng.normal(size=(nv * nc, 9))
+        y = ngw_ctx.matvec_isdf_BLOCK(x)
+        ref = O.matvec_block(psi, mm, O.sqrt_r(e1, nv, nc), x, nv, nc)
+        err = max(rel_err(y[:, j], ref[:, j]) for j in range(x.shape[1]))
+        print("odd shape", prec, nv, nc, err)
+        assert err < tol
+    finally:
+        ngw_ctx.set_precision("f64")
