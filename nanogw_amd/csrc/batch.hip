@@ -436,7 +436,7 @@ int rebuild_group(ngw_batch* b)
             for (int sgi = 0; sgi < npoly; ++sgi) {
                 const PolyStage& p = plan[sgi];
                 lstages[(size_t)i * npoly + sgi] = LpStage{p.x, p.out, p.y, p.alpha, p.bx, p.by, p.gamma,
-                                                           p.accumulate ? 2 : 0};
+                                                           p.accumulate ? 2 : 0, p.flags};
             }
             int tot[LP_NKERNEL];
             lp2_grid_counts(prec, b->lp_c1, b->lp_c3, d, tot);

@@ -165,7 +165,9 @@ std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const do
     }
     // Clenshaw on p(x) = sum c_k T_k(t), t = (x - be)/al:
     //   b_{n+1} = b_{n+2} = 0, b_k = 2 t(b_{k+1}) - b_{k+2} + c_k q, p = t(b_1) - b_2 + c_0 q.
-    // b_n = c_n q is kept implicit (a multiple of q).
+    // b_n = c_n q is kept implicit (a multiple of q).  The intermediates
+    // b_k are stored as FP32 in the h buffers (the GEMM operands are at most
+    // FP32 anyway); q and the result stay FP64.
     double lo, hi;
     fit_interval(ctx, lo, hi);
     const std::vector<double> c = chebyshev_from_monomial(poly, npoly, lo, hi);
@@ -181,11 +183,13 @@ std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const do
         double gam = c[k];
         if (bk1.p) {
             st.x = bk1.p; st.alpha = f / al; st.bx = -f * be / al;
+            st.flags |= ST_X32;
         } else {
             st.x = q; st.alpha = f * bk1.qc / al; gam += -f * be / al * bk1.qc;
         }
         if (bk2.p) {
             st.y = bk2.p; st.by = -1.0;
+            st.flags |= ST_Y32;
         } else {
             gam += -bk2.qc;
         }
@@ -200,6 +204,7 @@ std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const do
             nb = (nb + 1) % 3;
             if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
             if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
+            st.flags |= ST_O32;
         }
         st.out = o;
         plan.push_back(st);
@@ -221,6 +226,7 @@ int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, doubl
         a.v0 = q; a.ldv0 = ld; a.gamma = st.gamma;
         a.out = st.out; a.ldo = ld;
         if (st.accumulate) { a.acc = st.out; a.ldacc = ld; }
+        a.flags = st.flags;
         NGW_TRY(operator_apply(ctx, a));
     }
     return NGW_OK;

@@ -256,20 +256,26 @@ __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, 
     const long ncv = (long)op.nv * op.nc;
     const int split = xs_split(ncv);
     const int b = local / split, part = local - b * split;
-    const double* xb = (ops ? st[r * nstage + s].x : st1.x) + (long)b * op.ld;
+    const LpStage& sg = ops ? st[r * nstage + s] : st1;
+    const bool x32 = sg.flags & ST_X32;
+    const long xoff = (long)b * op.ld;
     const long chunk = (ncv + split - 1) / split;
     const long lo = part * chunk, hi = min(ncv, lo + chunk);
     double m = 0.0;
-    for (long base = lo + threadIdx.x; base < hi; base += 2048) {
-        double v[8];
+    auto scan = [&](auto* xp) {
+        for (long base = lo + threadIdx.x; base < hi; base += 2048) {
+            double v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const long idx = base + 256 * u;
-            v[u] = idx < hi ? fabs(xb[idx]) * (double)op.xsig[idx] : 0.0;
+            for (int u = 0; u < 8; ++u) {
+                const long idx = base + 256 * u;
+                v[u] = idx < hi ? fabs((double)xp[xoff + idx]) * (double)op.xsig[idx] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
-    }
+    };
+    if (x32) scan(reinterpret_cast<const float*>(sg.x));
+    else scan(sg.x);
     const float mb = block_max256((float)m);
     if (threadIdx.x == 0) op.xpart[b * XS_SPLIT + part] = mb;
 }
@@ -297,7 +303,8 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
     typedef typename V2Traits<KIND>::elem E;
     __shared__ float tile[XP_TILE];  // [c][v], pitch vt + 1
     LP_PICK_OP();
-    const double* x = ops ? st[r * nstage + s].x : st1.x;
+    const LpStage& sg = ops ? st[r * nstage + s] : st1;
+    const bool x32 = sg.flags & ST_X32;
     const int vt = xp_vt(op.nv), ct = xp_ct(op.nv, op.ldc1);
     const int nvt = (op.nv + vt - 1) / vt, nct = (op.ldc1 + ct - 1) / ct;
     const int b = local / (nvt * nct);
@@ -308,7 +315,7 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
     if (local == 0)  // the zero row after the last x' row
         for (int c = threadIdx.x; c < op.ldc1; c += 256)
             put_v2<KIND>((E*)op.xlp_hi, (E*)op.xlp_lo, (long)op.nv * op.nvec * op.ldc1 + c, 0.0f);
-    const double* xb = x + (long)b * op.ld;
+    const long xoff = (long)b * op.ld;
     double f16s = 1.0;
     if constexpr (KIND == LP_F16) {
         float m = 0.0f;
@@ -323,25 +330,29 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
     const int tot = vn * cn;
     const long base = v0 + (long)c0 * op.nv;
     const float inv = 1.0f / (float)vn;
-    for (int e0 = threadIdx.x; e0 < tot; e0 += 256 * 8) {
-        double xv[8], sv[8];
-        int cc[8], vv[8];
+    auto walk = [&](auto* xp) {
+        for (int e0 = threadIdx.x; e0 < tot; e0 += 256 * 8) {
+            double xv[8], sv[8];
+            int cc[8], vv[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + 256 * u;
-            int c = (int)((float)e * inv);
-            if (c * vn > e) --c;
-            if ((c + 1) * vn <= e) ++c;
-            cc[u] = c;
-            vv[u] = e - c * vn;
-            const long idx = base + vv[u] + (long)c * op.nv;
-            xv[u] = e < tot ? xb[idx] : 0.0;
-            sv[u] = e < tot ? (double)op.xsig[idx] : 0.0;
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 256 * u;
+                int c = (int)((float)e * inv);
+                if (c * vn > e) --c;
+                if ((c + 1) * vn <= e) ++c;
+                cc[u] = c;
+                vv[u] = e - c * vn;
+                const long idx = base + vv[u] + (long)c * op.nv;
+                xv[u] = e < tot ? (double)xp[xoff + idx] : 0.0;
+                sv[u] = e < tot ? (double)op.xsig[idx] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + 256 * u < tot) tile[cc[u] * pitch + vv[u]] = (float)(xv[u] * (sv[u] * f16s));
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (e0 + 256 * u < tot) tile[cc[u] * pitch + vv[u]] = (float)(xv[u] * (sv[u] * f16s));
-    }
+    };
+    if (x32) walk(reinterpret_cast<const float*>(sg.x));
+    else walk(sg.x);
     for (int i = cn * pitch + threadIdx.x; i < ct * pitch; i += 256) tile[i] = 0.0f;  // k padding rows
     __syncthreads();
     const int cpr = ct / 8;  // 8-wide chunks per row (a power of two)
@@ -583,28 +594,49 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                       OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0,
                                       n0, op.ldm1, acc, sm);
     }
-    OperatorEpilogue ep;
-    ep.x = sg.x; ep.ldx = op.ld;
-    ep.d = op.tda ? op.dtda : op.dfull;
-    ep.s = op.tda ? nullptr : op.sqrtr;
-    ep.beta = op.tda ? 2.0 : 4.0;
-    ep.alpha = sg.alpha;
-    ep.bx = sg.bx;
-    ep.y = sg.y; ep.ldy = op.ld; ep.by = sg.by;
-    ep.v0 = op.q; ep.ldv0 = op.ld; ep.gamma = sg.gamma;
+    // epilogue: OperatorEpilogue's stage (diagonal term, Horner / Clenshaw
+    // combination, Lanczos accumulate), written out for one leading
+    // dimension shared by every vector block of the request
+    const double* dd = op.tda ? op.dtda : op.dfull;
+    const double* ss = op.tda ? nullptr : op.sqrtr;
+    const double beta = op.tda ? 2.0 : 4.0;
     const bool acc_on = sg.accumulate == 1 || (sg.accumulate == 2 && *op.kstep > 1);
-    ep.acc = acc_on ? sg.out : nullptr; ep.ldacc = op.ld;
-    ep.out = sg.out; ep.ldo = op.ld;
+    const bool use_q = op.q != nullptr && sg.gamma != 0.0;
     const float* sigc = op.sig + nv;
-    for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
-        int c = m0 + row, j = n0 + col;
-        if (c < op.nc && j < N) {
-            int b = j / nv, v = j - b * nv;
-            double y = (double)val * (double)sigc[c];
-            if constexpr (KIND == LP_F16) y *= (double)op.wsc[j] * F16_UNSCALE;
-            ep.apply(v + (long)c * nv, b, y);
-        }
-    });
+    // element types of x, y, out fixed at compile time per flag combination
+    auto epi = [&](auto xt, auto yt, auto ot) {
+        typedef decltype(xt) TX;
+        typedef decltype(yt) TY;
+        typedef decltype(ot) TO;
+        const TX* xp = reinterpret_cast<const TX*>(sg.x);
+        const TY* yp = reinterpret_cast<const TY*>(sg.y);
+        TO* op_ = reinterpret_cast<TO*>(sg.out);
+        for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
+            const int c = m0 + row, j = n0 + col;
+            if (c < op.nc && j < N) {
+                const int b = j / nv, v = j - b * nv;
+                const long idx = v + (long)c * nv, i = idx + (long)b * op.ld;
+                double Y = (double)val * (double)sigc[c];
+                if constexpr (KIND == LP_F16) Y *= (double)op.wsc[j] * F16_UNSCALE;
+                const double sv = ss ? ss[idx] : 1.0;
+                const double xv = (double)xp[i];
+                const double hx = dd[idx] * (ss ? sv * xv : xv) + (beta * sv) * Y;
+                double r = sg.alpha * hx + sg.bx * xv;
+                if (yp) r = r + sg.by * (double)yp[i];
+                if (use_q) r = r + sg.gamma * op.q[i];
+                if (acc_on) r = (double)op_[i] + r;
+                op_[i] = (TO)r;
+            }
+        });
+    };
+    switch (sg.flags & (ST_X32 | ST_Y32 | ST_O32)) {
+        case 0: epi(double(), double(), double()); break;
+        case ST_O32: epi(double(), double(), float()); break;
+        case ST_X32 | ST_O32: epi(float(), double(), float()); break;
+        case ST_X32 | ST_Y32 | ST_O32: epi(float(), float(), float()); break;
+        case ST_X32 | ST_Y32: epi(float(), float(), double()); break;
+        default: epi(float(), float(), double()); break;  // not produced by the stage planner
+    }
 }
 
 // ---------------------------------------------------------------- launch plumbing
@@ -793,7 +825,7 @@ int operator_lp2(ngw_context* ctx, const OperatorArgs& a)
     if (a.ldo != a.ldx || (a.v0 && a.ldv0 != a.ldx) || (a.y && a.ldy != a.ldx) || (a.acc && a.acc != a.out))
         return fail(NGW_ERR_ARG, "low-precision operator needs one leading dimension and acc == out");
     d.q = const_cast<double*>(a.v0);
-    LpStage st{a.x, a.out, a.y, a.alpha, a.bx, a.by, a.v0 ? a.gamma : 0.0, a.acc ? 1 : 0};
+    LpStage st{a.x, a.out, a.y, a.alpha, a.bx, a.by, a.v0 ? a.gamma : 0.0, a.acc ? 1 : 0, a.flags};
     const int c1 = ctx->lp_cfg[kind][0], c3 = ctx->lp_cfg[kind][1];
     int tot[LP_NKERNEL];
     lp2_grid_counts(kind, c1, c3, d, tot);

@@ -5,12 +5,29 @@
 
 namespace ngw {
 
+// Vector blocks of the polynomial evaluation are FP64, or FP32 for the
+// Clenshaw intermediates of the low-precision modes (flag set): the element
+// i of a block stored as float is ((const float*)p)[i].
+__device__ __forceinline__ double vload(const double* p, bool f32, long i)
+{
+    return f32 ? (double)reinterpret_cast<const float*>(p)[i] : p[i];
+}
+__device__ __forceinline__ void vstore(double* p, bool f32, long i, double v)
+{
+    if (f32) reinterpret_cast<float*>(p)[i] = (float)v;
+    else p[i] = v;
+}
+
+// Stage flags: which vector blocks of a stage are stored as FP32.
+enum { ST_X32 = 1, ST_Y32 = 2, ST_O32 = 4 };
+
 // Epilogue of the operator's last GEMM (K3), applied per output element
 // (pair index idx of vector b, GEMM value Y = (C^T M C x')(idx)):
 //   hx  = d(idx) x'(idx) + beta s(idx) Y        (x' = s o x, or x if s == null)
 //   r   = alpha hx + bx x(idx) + by y(idx) + gamma v0(idx)
 //                                                (one Horner or Clenshaw stage)
 //   out = acc(idx) + r                           (Lanczos w += p(H) q)
+// x, y and out (and acc, which is out) may be FP32 blocks (flags).
 struct OperatorEpilogue {
     const double* x = nullptr; long ldx = 0;
     const double* d = nullptr;
@@ -21,6 +38,7 @@ struct OperatorEpilogue {
     double* out = nullptr; long ldo = 0;
     double bx = 0.0;
     const double* y = nullptr; long ldy = 0; double by = 0.0;
+    int flags = 0;
 
     __device__ __forceinline__ void apply(long idx, int b, double Y) const
     {
@@ -28,16 +46,16 @@ struct OperatorEpilogue {
         double hx = (beta * sv) * Y;
         double xv = 0.0;
         if (x) {
-            xv = x[idx + (long)b * ldx];
+            xv = vload(x, flags & ST_X32, idx + (long)b * ldx);
             double xp = s ? sv * xv : xv;
             hx = d[idx] * xp + hx;
         }
         double r = alpha * hx;
         if (bx != 0.0) r = r + bx * xv;
-        if (y) r = r + by * y[idx + (long)b * ldy];
+        if (y) r = r + by * vload(y, flags & ST_Y32, idx + (long)b * ldy);
         if (v0) r = r + gamma * v0[idx + (long)b * ldv0];
-        if (acc) r = acc[idx + (long)b * ldacc] + r;
-        out[idx + (long)b * ldo] = r;
+        if (acc) r = vload(acc, flags & ST_O32, idx + (long)b * ldacc) + r;
+        vstore(out, flags & ST_O32, idx + (long)b * ldo, r);
     }
 };
 
@@ -53,6 +71,7 @@ struct OperatorArgs {
     const double* acc = nullptr; long ldacc = 0;
     double bx = 0.0;
     const double* y = nullptr; long ldy = 0; double by = 0.0;
+    int flags = 0;  // ST_*: FP32 vector blocks (low-precision Clenshaw intermediates)
 
     OperatorEpilogue epilogue(const ngw_context* ctx) const
     {
@@ -66,6 +85,7 @@ struct OperatorArgs {
         e.acc = acc; e.ldacc = ldacc;
         e.out = out; e.ldo = ldo;
         e.bx = bx; e.y = y; e.ldy = ldy; e.by = by;
+        e.flags = flags;
         return e;
     }
 };
@@ -86,6 +106,7 @@ struct PolyStage {
     const double* y = nullptr;
     double alpha = 1.0, bx = 0.0, by = 0.0, gamma = 0.0;
     bool accumulate = false;
+    int flags = 0;  // ST_*
 };
 std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const double* q, double* out, double* h0,
                                  double* h1, double* h2, const double* poly, int npoly, bool accumulate);
@@ -130,6 +151,7 @@ struct LpStage {
     const double* x; double* out; const double* y;
     double alpha, bx, by, gamma;
     int accumulate;
+    int flags;  // ST_*
 };
 
 int lp2_prepare(ngw_context* ctx);
