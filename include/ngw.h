@@ -79,6 +79,25 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
 int ngw_set_precision(ngw_context* ctx, int precision);
 
 /*
+ * Upload the operator in the explicit form of NanoGW's single-vector
+ * routines, matvec_isdf (shared/apply_matvec.f90:25), polynomial_matvec_isdf
+ * (:142) and lanczos_spectra_isdf (shared/lanczos_spectra.f90:7), which take
+ * the pair matrix and the energy factors as arguments:
+ *   cmtrx(ld_c, ncv)      tmpCmtrx(n_intp_r, ncv_loc), C(mu, icv) =
+ *                         Psi_v(mu) Psi_c(mu) for any pair list
+ *   mmtrx(ld_m, n_intp)   isdf_in%Mmtrx(:, :, 1, 1, 1, 1, 1)
+ *   sqrtr(ncv)            sqrtR, (e_c - e_v)^1/2 > 0
+ * After it the operator entries (ngw_matvec_isdf_block,
+ * ngw_polynomial_matvec_isdf_block, ngw_lanczos_spectra_isdf_block and the
+ * _dev apply) compute the same y = R^2 x + 4 R^1/2 C^T M C R^1/2 x (TDA:
+ * y = R x + 2 C^T M C x) with the three products as FP64 library GEMMs.
+ * FP64 only; the Sigma_c / Sigma_x entries and ngw_autotune need the
+ * factored operator and fail with NGW_ERR_STATE.  ngw_set_isdf switches back.
+ */
+int ngw_set_explicit_c(ngw_context* ctx, int n_intp, int ncv, const double* cmtrx, int ld_c,
+                       const double* mmtrx, int ld_m, const double* sqrtr);
+
+/*
  * Operator apply on blksz vectors, replaces matvec_isdf_BLOCK
  * (shared/apply_matvec_BLOCK.F90:25) and, with tamm_d = 1, the TDA branch of
  * matvec_isdf_lowcomsym (shared/apply_matvec_lowcomsym.F90:31, :396-403):
@@ -106,8 +125,9 @@ int ngw_polynomial_matvec_isdf_block(ngw_context* ctx, const double* vec, double
  *   v0(ncv, blksz), v0_norm(blksz), zz(nz, blksz) complex,
  *   spectra(nz, blksz) complex (output).
  * a_elmt, b_elmt (niter, blksz) receive the recurrence coefficients when not
- * NULL.  Also replaces lanczos_spectra_isdf (shared/lanczos_spectra.f90:7)
- * with blksz = 1.
+ * NULL.  After ngw_set_explicit_c, blksz = 1 is lanczos_spectra_isdf
+ * (shared/lanczos_spectra.f90:7) but for its breakdown exit at |b_k| < 1e-8
+ * (:58), which the caller applies to the returned coefficients.
  */
 int ngw_lanczos_spectra_isdf_block(ngw_context* ctx, const double* v0, const double* v0_norm,
                                    int ncv, const double* zz, int nz, int niter,

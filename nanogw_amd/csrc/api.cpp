@@ -90,6 +90,7 @@ static void prof_drain(ngw_context* c)
 
 int operator_apply(ngw_context* ctx, const OperatorArgs& a)
 {
+    if (ctx->explicit_c) return operator_explicit(ctx, a);
     if (ctx->precision == PREC_F64) return operator_f64(ctx, a);
     return operator_lp2(ctx, a);
 }
@@ -99,6 +100,15 @@ static int check_ctx(ngw_context* ctx)
     if (!ctx) return fail(NGW_ERR_ARG, "null context");
     if (ctx->n_intp <= 0 || ctx->ncv <= 0) return fail(NGW_ERR_STATE, "ISDF operator not uploaded (ngw_set_isdf)");
     NGW_HIP(hipSetDevice(ctx->device));
+    return NGW_OK;
+}
+
+// Entries that read the orbitals at the ISDF points, not only the operator.
+static int check_factored(ngw_context* ctx)
+{
+    NGW_TRY(check_ctx(ctx));
+    if (ctx->explicit_c)
+        return fail(NGW_ERR_STATE, "operator given by an explicit C matrix: this entry needs ngw_set_isdf");
     return NGW_OK;
 }
 
@@ -231,6 +241,7 @@ int ngw_destroy(ngw_context* ctx)
                       &ctx->pct32, &ctx->pc32, &ctx->pctb, &ctx->pcb, &ctx->pcth, &ctx->pch, &ctx->m32t,
                       &ctx->psiv32, &ctx->psiv_scale, &ctx->xsig, &ctx->lpwork};
     for (DevBuf* b : bufs) b->release();
+    explicit_release(ctx);
     for (auto& r : ctx->prof_recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -257,6 +268,7 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
     if (ld_psi < n_intp || ld_m < n_intp) return fail(NGW_ERR_ARG, "leading dimension < n_intp");
     if (!psi_intp || !mmtrx || !e1) return fail(NGW_ERR_ARG, "null input array");
     NGW_HIP(hipSetDevice(ctx->device));
+    explicit_release(ctx);
     const int nc = nband - nval;
     const long ncv = (long)nval * nc;
     for (int ic = 0; ic < nc; ++ic)
@@ -306,6 +318,8 @@ int ngw_set_precision(ngw_context* ctx, int precision)
 {
     if (!ctx) return fail(NGW_ERR_ARG, "null context");
     if (precision < NGW_PREC_F64 || precision > NGW_PREC_F16) return fail(NGW_ERR_ARG, "bad precision");
+    if (ctx->explicit_c && precision != NGW_PREC_F64)
+        return fail(NGW_ERR_STATE, "the explicit-C operator is FP64 only");
     ctx->precision = precision;
     if (precision != NGW_PREC_F64 && ctx->n_intp > 0) {
         NGW_HIP(hipSetDevice(ctx->device));
@@ -329,7 +343,7 @@ int ngw_matvec_isdf_block_dev(ngw_context* ctx, const double* d_vec, double* d_h
 
 int ngw_autotune(ngw_context* ctx, int nvec, int reps, int* k1_cfg, int* k3_cfg)
 {
-    NGW_TRY(check_ctx(ctx));
+    NGW_TRY(check_factored(ctx));
     if (nvec <= 0 || reps <= 0) return fail(NGW_ERR_ARG, "need nvec > 0 and reps > 0");
     const int prec = ctx->precision;
     if (prec == PREC_F64) {
@@ -393,7 +407,7 @@ int ngw_matvec_isdf_block(ngw_context* ctx, const double* vec, double* hvec, int
                           int tamm_d)
 {
     NGW_TRY(check_ctx(ctx));
-    if (ncv != ctx->ncv) return fail(NGW_ERR_ARG, "ncv does not match nval*ncond of the operator");
+    if (ncv != ctx->ncv) return fail(NGW_ERR_ARG, "ncv does not match the operator (nval*ncond, or the columns of C)");
     if (blksz < 0 || (blksz > 0 && (!vec || !hvec))) return fail(NGW_ERR_ARG, "bad vectors");
     if (blksz == 0) return NGW_OK;
     const size_t bytes = sizeof(double) * (size_t)ncv * blksz;
@@ -410,7 +424,7 @@ int ngw_polynomial_matvec_isdf_block(ngw_context* ctx, const double* vec, double
                                      const double* polynomial, int npoly, int blksz, int tamm_d)
 {
     NGW_TRY(check_ctx(ctx));
-    if (ncv != ctx->ncv) return fail(NGW_ERR_ARG, "ncv does not match nval*ncond of the operator");
+    if (ncv != ctx->ncv) return fail(NGW_ERR_ARG, "ncv does not match the operator (nval*ncond, or the columns of C)");
     if (npoly < 1 || !polynomial) return fail(NGW_ERR_ARG, "npoly must be >= 1");
     if (blksz < 0 || (blksz > 0 && (!vec || !hvec))) return fail(NGW_ERR_ARG, "bad vectors");
     if (blksz == 0) return NGW_OK;
@@ -436,7 +450,7 @@ int ngw_lanczos_spectra_isdf_block(ngw_context* ctx, const double* v0, const dou
                                    int tamm_d, double* a_elmt, double* b_elmt)
 {
     NGW_TRY(check_ctx(ctx));
-    if (ncv != ctx->ncv) return fail(NGW_ERR_ARG, "ncv does not match nval*ncond of the operator");
+    if (ncv != ctx->ncv) return fail(NGW_ERR_ARG, "ncv does not match the operator (nval*ncond, or the columns of C)");
     if (npoly < 1 || !polynomial) return fail(NGW_ERR_ARG, "npoly must be >= 1");
     if (niter < 1 || nz < 0 || blksz < 0) return fail(NGW_ERR_ARG, "bad niter, nz or blksz");
     if (blksz == 0) return NGW_OK;
@@ -483,7 +497,7 @@ int ngw_sigma_lanczos(ngw_context* ctx, int nqp, const int* qp_band, int nen, do
                       double ecuts, int niter, const double* polynomial, int npoly, int tamm_d,
                       int max_cols, double* sigma_c)
 {
-    NGW_TRY(check_ctx(ctx));
+    NGW_TRY(check_factored(ctx));
     return ngw_sigma_lanczos_bands(ctx, nqp, qp_band, 0, ctx->nband, nen, deltae, ecuts, niter, polynomial, npoly,
                                    tamm_d, max_cols, sigma_c);
 }
@@ -492,7 +506,7 @@ int ngw_sigma_lanczos_bands(ngw_context* ctx, int nqp, const int* qp_band, int b
                             double deltae, double ecuts, int niter, const double* polynomial, int npoly, int tamm_d,
                             int max_cols, double* sigma_c)
 {
-    NGW_TRY(check_ctx(ctx));
+    NGW_TRY(check_factored(ctx));
     if (nqp <= 0 || !qp_band || !sigma_c) return fail(NGW_ERR_ARG, "need nqp > 0 and arrays");
     if (nen < 2 || niter < 1 || npoly < 1 || !polynomial) return fail(NGW_ERR_ARG, "bad nen, niter or npoly");
     if (band_lo < 0 || band_hi > ctx->nband || band_lo >= band_hi)
@@ -570,7 +584,7 @@ int ngw_sigma_lanczos_bands(ngw_context* ctx, int nqp, const int* qp_band, int b
 
 int ngw_sigma_x_isdf(ngw_context* ctx, int nqp, const int* qp_band, double* sigx)
 {
-    NGW_TRY(check_ctx(ctx));
+    NGW_TRY(check_factored(ctx));
     if (nqp <= 0 || !qp_band || !sigx) return fail(NGW_ERR_ARG, "need nqp > 0 and arrays");
     for (int i = 0; i < nqp; ++i)
         if (qp_band[i] < 0 || qp_band[i] >= ctx->nband) return fail(NGW_ERR_ARG, "qp_band out of range");

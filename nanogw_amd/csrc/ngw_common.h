@@ -90,6 +90,11 @@ struct ngw_context {
     ngw::DevBuf pct32, pc32, pctb, pcb, pcth, pch, m32t, psiv32, psiv_scale, xsig, lpwork;
     // chosen {K1, K3} tile configuration per precision mode (ngw_autotune)
     int lp_cfg[ngw::PREC_COUNT][2] = {};
+    // explicit pair matrix C (n_intp x ncv, ngw_set_explicit_c) in place of
+    // the factored Psi_v o Psi_c; its rocBLAS handle and GEMM workspaces
+    bool explicit_c = false;
+    ngw::DevBuf cexp, ex_x, ex_y, ex_t;
+    void* rb_handle = nullptr;
     // workspaces
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;
     // host staging for drop-in calls

@@ -120,7 +120,11 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a);
 int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int nvec, int tda,
                  double* d_p, long ldp_out);
 
-// Dispatch on ctx->precision.
+// The operator with an explicit pair matrix C (rocBLAS products, FP64).
+int operator_explicit(ngw_context* ctx, const OperatorArgs& a);
+void explicit_release(ngw_context* ctx);
+
+// Dispatch on the operator form and ctx->precision.
 int operator_apply(ngw_context* ctx, const OperatorArgs& a);
 int lowp_prepare(ngw_context* ctx);
 // Low-precision request descriptor (the operator of one system plus one

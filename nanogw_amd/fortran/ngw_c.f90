@@ -46,6 +46,14 @@ module ngw_c
       integer(c_int), value :: precision
     end function ngw_set_precision
 
+    integer(c_int) function ngw_set_explicit_c(ctx, n_intp, ncv, cmtrx, ld_c, mmtrx, ld_m, sqrtr) &
+      bind(C, name='ngw_set_explicit_c')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_intp, ncv, ld_c, ld_m
+      real(c_double), intent(in) :: cmtrx(*), mmtrx(*), sqrtr(*)
+    end function ngw_set_explicit_c
+
     integer(c_int) function ngw_matvec_isdf_block(ctx, vec, hvec, ncv, blksz, tamm_d) &
       bind(C, name='ngw_matvec_isdf_block')
       import :: c_ptr, c_int, c_double

@@ -5,14 +5,22 @@
 ! shared/lanczos_spectra.f90: the external subroutines keep the reference's
 ! names and argument lists (implicit interfaces, arguments by reference), so
 ! Zcalculate_sigma_lanczos_BLOCK and Zcalculate_sigma_lanczos call them
-! unchanged.  The operator (Psi at the ISDF points, Mmtrx, energies) is
-! uploaded to the GPU once and kept resident; it is re-uploaded when the
-! caller passes a different array or size.
+! unchanged.
 !
-! Scope: one MPI rank per GPU holding the whole transition space
-! (w_grp%npes = 1), all nval x ncond pairs in NanoGW's conduction-outer /
-! valence-inner order (shared/setup_g.F90z:154-168); other layouts stop with
-! a message, as NanoGW's die() would.
+! BLOCK routines: the operator (Psi at the ISDF points, Mmtrx, energies) is
+! uploaded to the GPU once and kept resident; it is re-uploaded when the
+! caller passes a different array or size.  Scope: one MPI rank per GPU
+! holding the whole transition space (w_grp%npes = 1), all nval x ncond pairs
+! in NanoGW's conduction-outer / valence-inner order
+! (shared/setup_g.F90z:154-168); other layouts stop with a message, as
+! NanoGW's die() would.
+!
+! Single-vector routines (matvec_isdf, polynomial_matvec_isdf,
+! lanczos_spectra_isdf): the caller's tmpCmtrx, Mmtrx and sqrtR are uploaded
+! as an explicit operator (any pair list) and kept resident while the same
+! arrays, sizes, sqrtR and a sample of Cmtrx and Mmtrx are passed again.
+! One MPI rank (peinf%npes = 1); FP64 whatever precision the BLOCK routines
+! use.
 !
 ! Built inside the NanoGW tree it uses NanoGW's typedefs and mpi_module; with
 ! -DNGW_DROPIN_SELFTEST it compiles against ngw_dropin_selftest_types, a
@@ -35,8 +43,63 @@ module ngw_session
   integer, save :: up_nintp = -1, up_nband = -1, up_nval = -1
   integer, save :: ngw_device = 0
   integer, save :: ngw_precision = 0
+  ! explicit-operator state of the single-vector routines
+  integer, parameter :: ngw_nsample = 2048
+  type(c_ptr), save :: up_c = c_null_ptr, up_mx = c_null_ptr
+  integer, save :: up_cnintp = -1, up_cncv = -1
+  real(c_double), allocatable, save :: up_sqrtr(:), up_csamp(:), up_msamp(:)
 
 contains
+
+  subroutine ngw_start()
+    if (.not. c_associated(ngw_ctx)) then
+      call ngw_check(ngw_create(ngw_ctx, int(ngw_device, c_int)), 'ngw_create')
+      call ngw_check(ngw_set_precision(ngw_ctx, int(ngw_precision, c_int)), 'ngw_set_precision')
+    end if
+  end subroutine ngw_start
+
+  ! Evenly spaced entries of a(1:n).
+  subroutine ngw_sample(a, n, out)
+    integer, intent(in) :: n
+    real(c_double), intent(in) :: a(n)
+    real(c_double), allocatable, intent(out) :: out(:)
+    integer :: m, i, stride
+    m = min(n, ngw_nsample)
+    stride = max(1, n / max(m, 1))
+    allocate (out(m))
+    do i = 1, m
+      out(i) = a(1 + (i - 1) * stride)
+    end do
+  end subroutine ngw_sample
+
+  ! Upload tmpCmtrx(n_intp, ncv), Mmtrx(ld_m, n_intp) and sqrtR(ncv) as the
+  ! explicit operator unless the same ones are resident.
+  subroutine ngw_ensure_explicit(n_intp, ncv, cmtrx, ld_m, mmtrx, sqrtr)
+    integer, intent(in) :: n_intp, ncv, ld_m
+    real(c_double), intent(in), target :: cmtrx(n_intp, ncv), mmtrx(ld_m, n_intp)
+    real(c_double), intent(in) :: sqrtr(ncv)
+    real(c_double), allocatable :: cs(:), ms(:)
+    call ngw_start()
+    call ngw_sample(cmtrx, n_intp * ncv, cs)
+    call ngw_sample(mmtrx, ld_m * n_intp, ms)
+    if (c_associated(up_c, c_loc(cmtrx(1, 1))) .and. c_associated(up_mx, c_loc(mmtrx(1, 1))) .and. &
+        up_cnintp == n_intp .and. up_cncv == ncv) then
+      if (all(up_sqrtr == sqrtr) .and. all(up_csamp == cs) .and. all(up_msamp == ms)) return
+    end if
+    call ngw_check(ngw_set_explicit_c(ngw_ctx, int(n_intp, c_int), int(ncv, c_int), cmtrx, int(n_intp, c_int), &
+                                      mmtrx, int(ld_m, c_int), sqrtr), 'ngw_set_explicit_c')
+    up_c = c_loc(cmtrx(1, 1))
+    up_mx = c_loc(mmtrx(1, 1))
+    up_cnintp = n_intp
+    up_cncv = ncv
+    up_sqrtr = sqrtr
+    call move_alloc(cs, up_csamp)
+    call move_alloc(ms, up_msamp)
+    ! the factored operator is gone from the context
+    up_psi = c_null_ptr
+    up_m = c_null_ptr
+    up_nintp = -1
+  end subroutine ngw_ensure_explicit
 
   ! nval and nband from the pair list; checks NanoGW's all-pairs ordering.
   subroutine ngw_pairs(cvpair, ldc, ncv, nval, nband)
@@ -65,10 +128,7 @@ contains
     integer, intent(in) :: n_intp, ld_psi, ld_m, nband, nval
     real(c_double), intent(in), target :: psi(ld_psi, *), mmtrx(ld_m, *)
     real(c_double), intent(in) :: e1(*)
-    if (.not. c_associated(ngw_ctx)) then
-      call ngw_check(ngw_create(ngw_ctx, int(ngw_device, c_int)), 'ngw_create')
-      call ngw_check(ngw_set_precision(ngw_ctx, int(ngw_precision, c_int)), 'ngw_set_precision')
-    end if
+    call ngw_start()
     if (c_associated(up_psi, c_loc(psi(1, 1))) .and. c_associated(up_m, c_loc(mmtrx(1, 1))) .and. &
         up_nintp == n_intp .and. up_nband == nband .and. up_nval == nval) return
     call ngw_check(ngw_set_isdf(ngw_ctx, int(n_intp, c_int), int(nband, c_int), int(nval, c_int), &
@@ -79,6 +139,8 @@ contains
     up_nintp = n_intp
     up_nband = nband
     up_nval = nval
+    up_c = c_null_ptr
+    up_cnintp = -1
   end subroutine ngw_ensure
 
   subroutine ngw_release()
@@ -88,6 +150,9 @@ contains
     up_psi = c_null_ptr
     up_m = c_null_ptr
     up_nintp = -1
+    up_c = c_null_ptr
+    up_mx = c_null_ptr
+    up_cnintp = -1
   end subroutine ngw_release
 
 end module ngw_session
@@ -165,3 +230,90 @@ subroutine lanczos_spectra_isdf_BLOCK(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_
                                                 int(blksz, c_int), 0_c_int, c_null_ptr, c_null_ptr), &
                  'lanczos_spectra_isdf_BLOCK')
 end subroutine lanczos_spectra_isdf_BLOCK
+
+! shared/apply_matvec.f90:25
+subroutine matvec_isdf(vec, Hvec, ncv_loc, isdf_in, wfn, tmpCmtrx, sqrtR)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc
+  real(dp), intent(in) :: vec(ncv_loc)
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(in) :: tmpCmtrx(isdf_in%n_intp_r, ncv_loc), sqrtR(ncv_loc)
+  real(dp), intent(out) :: Hvec(ncv_loc)
+  type(wavefunction), intent(in) :: wfn
+  if (peinf%npes /= 1) stop 'libngw drop-in: one rank per GPU (peinf%npes = 1)'
+  call ngw_ensure_explicit(isdf_in%n_intp_r, ncv_loc, tmpCmtrx, size(isdf_in%Mmtrx, 1), &
+                           isdf_in%Mmtrx(:, :, 1, 1, 1, 1, 1), sqrtR)
+  call ngw_check(ngw_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), 1_c_int, 0_c_int), 'matvec_isdf')
+end subroutine matvec_isdf
+
+! shared/apply_matvec.f90:142 (tmp_Cmtrx is the (n_intp_r, ncv_loc) matrix the
+! reference passes through to matvec_isdf)
+subroutine polynomial_matvec_isdf(vec, Hvec, ncv_loc, polynomial, npoly, isdf_in, wfn, tmp_Cmtrx, sqrtR)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, npoly
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(in) :: polynomial(npoly + 1), tmp_Cmtrx(isdf_in%n_intp_r, ncv_loc), sqrtR(ncv_loc)
+  real(dp), intent(in) :: vec(ncv_loc)
+  type(wavefunction), intent(in) :: wfn
+  real(dp), intent(out) :: Hvec(ncv_loc)
+  if (peinf%npes /= 1) stop 'libngw drop-in: one rank per GPU (peinf%npes = 1)'
+  call ngw_ensure_explicit(isdf_in%n_intp_r, ncv_loc, tmp_Cmtrx, size(isdf_in%Mmtrx, 1), &
+                           isdf_in%Mmtrx(:, :, 1, 1, 1, 1, 1), sqrtR)
+  call ngw_check(ngw_polynomial_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), polynomial, &
+                                                  int(npoly, c_int), 1_c_int, 0_c_int), 'polynomial_matvec_isdf')
+end subroutine polynomial_matvec_isdf
+
+! shared/lanczos_spectra.f90:7.  The reference leaves the recurrence at the
+! first |b_k| < 1e-8 (:58) and then reads a(k..niter), b(k+1..niter) unset
+! in the continued fraction; here the fraction ends at level k - 1, its
+! exact-arithmetic value.  Without a breakdown the spectra are the library's.
+subroutine lanczos_spectra_isdf(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_in, &
+                                wfn, tmp_Cmtrx, sqrtR, polynomial, npoly, spectra)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, niter, nz, npoly
+  complex(dp), intent(in) :: zz(nz)
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(in) :: v0(ncv_loc), polynomial(npoly + 1), v0_norm, &
+                          tmp_Cmtrx(isdf_in%n_intp_r, ncv_loc), sqrtR(ncv_loc)
+  complex(dp), intent(out) :: spectra(nz)
+  type(wavefunction), intent(in) :: wfn
+  real(c_double), allocatable, target :: a_elmt(:), b_elmt(:)
+  real(c_double) :: vn(1)
+  real(dp), parameter :: eps = 1.0d-8
+  complex(dp) :: s, enum
+  integer :: k, j, iz, levels
+  if (peinf%npes /= 1) stop 'libngw drop-in: one rank per GPU (peinf%npes = 1)'
+  call ngw_ensure_explicit(isdf_in%n_intp_r, ncv_loc, tmp_Cmtrx, size(isdf_in%Mmtrx, 1), &
+                           isdf_in%Mmtrx(:, :, 1, 1, 1, 1, 1), sqrtR)
+  allocate (a_elmt(niter), b_elmt(niter))
+  vn(1) = v0_norm
+  call ngw_check(ngw_lanczos_spectra_isdf_block(ngw_ctx, v0, vn, int(ncv_loc, c_int), zz, int(nz, c_int), &
+                                                int(niter, c_int), polynomial, int(npoly, c_int), spectra, &
+                                                1_c_int, 0_c_int, c_loc(a_elmt), c_loc(b_elmt)), &
+                 'lanczos_spectra_isdf')
+  levels = niter
+  do k = 2, niter
+    if (abs(b_elmt(k)) < eps) then
+      levels = k - 1
+      exit
+    end if
+  end do
+  if (levels < niter) then
+    do iz = 1, nz
+      s = (0.d0, 0.d0)
+      do j = levels, 1, -1
+        enum = b_elmt(j) * b_elmt(j)
+        if (j == 1) enum = (1.d0, 0.d0)
+        s = enum / (zz(iz) - a_elmt(j) - s)
+      end do
+      spectra(iz) = s
+    end do
+  end if
+  deallocate (a_elmt, b_elmt)
+end subroutine lanczos_spectra_isdf

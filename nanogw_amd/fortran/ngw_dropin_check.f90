@@ -1,8 +1,11 @@
 ! Compile-and-call check of ngw_dropin.F90 outside NanoGW (see
 ! ngw_dropin_selftest_types.f90): reads a small problem from a stream file
 ! (argument 1), calls the drop-in matvec_isdf_BLOCK and
-! lanczos_spectra_isdf_BLOCK as Zcalculate_sigma_lanczos_BLOCK does, and
-! writes Hvec and the spectra to a stream file (argument 2).
+! lanczos_spectra_isdf_BLOCK as Zcalculate_sigma_lanczos_BLOCK does, then the
+! single-vector matvec_isdf, polynomial_matvec_isdf and lanczos_spectra_isdf
+! as Zcalculate_sigma_lanczos does (tmp_Cmtrx built from Psi, here over the
+! pair list in reverse order), and writes the results to a stream file
+! (argument 2).
 program ngw_dropin_check
   use ngw_dropin_selftest_types
   use ngw_session, only: ngw_release
@@ -12,7 +15,9 @@ program ngw_dropin_check
   integer :: nmu, nv, nc, nb, ncv, blksz, nz, niter, npoly, icv, u
   integer, allocatable :: cvpair(:, :), tmp_cvpair(:)
   real(dp), allocatable :: vec(:, :), hvec(:, :), sqrtR(:), poly(:), v0norm(:), dummy(:)
-  complex(dp), allocatable :: zz(:, :), spectra(:, :)
+  real(dp), allocatable :: cmtrx(:, :), sqrtr1(:), v1(:), h1(:), p1(:), veig(:)
+  complex(dp), allocatable :: zz(:, :), spectra(:, :), spec1(:), speig(:)
+  integer :: iv, ic, jcv
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -24,9 +29,12 @@ program ngw_dropin_check
   allocate (isdf_in%Psi_intp_loc(nmu, nb, 1, 1), isdf_in%Mmtrx_loc(nmu, nmu, 1, 1, 1, 1, 1), wfn%e1(nb))
   allocate (vec(ncv, blksz), hvec(ncv, blksz), sqrtR(ncv), poly(npoly + 1), v0norm(blksz))
   allocate (zz(nz, blksz), spectra(nz, blksz), cvpair(4, ncv), tmp_cvpair(2 * ncv), dummy(1))
-  read (u) isdf_in%Psi_intp_loc, isdf_in%Mmtrx_loc, wfn%e1, vec, poly, zz
+  allocate (isdf_in%Mmtrx(nmu, nmu, 1, 1, 1, 1, 1), cmtrx(nmu, ncv), sqrtr1(ncv), v1(ncv), h1(ncv), p1(ncv))
+  allocate (veig(ncv), spec1(nz), speig(nz))
+  read (u) isdf_in%Psi_intp_loc, isdf_in%Mmtrx_loc, wfn%e1, vec, poly, zz, veig
   close (u)
   isdf_in%n_intp_r = nmu
+  isdf_in%Mmtrx = isdf_in%Mmtrx_loc
   ! NanoGW's pair list: conduction outer, valence inner (setup_g.F90z:154-168)
   do icv = 1, ncv
     cvpair(1, icv) = mod(icv - 1, nv) + 1
@@ -39,8 +47,23 @@ program ngw_dropin_check
   v0norm = sqrt(sum(vec**2, dim=1))
   call lanczos_spectra_isdf_BLOCK(vec, v0norm, ncv, zz, nz, niter, isdf_in, wfn, cvpair, ncv, sqrtR, poly, &
                                   npoly, spectra, dummy, dummy, dummy, dummy, tmp_cvpair, blksz)
+  ! single-vector routines: pair jcv = ncv + 1 - icv
+  do icv = 1, ncv
+    jcv = ncv + 1 - icv
+    iv = cvpair(1, icv)
+    ic = cvpair(2, icv)
+    sqrtr1(jcv) = sqrt(wfn%e1(ic) - wfn%e1(iv))
+    cmtrx(:, jcv) = isdf_in%Psi_intp_loc(:, ic, 1, 1) * isdf_in%Psi_intp_loc(:, iv, 1, 1)
+    v1(jcv) = vec(icv, 1)
+  end do
+  call matvec_isdf(v1, h1, ncv, isdf_in, wfn, cmtrx, sqrtr1)
+  call polynomial_matvec_isdf(v1, p1, ncv, poly, npoly, isdf_in, wfn, cmtrx, sqrtr1)
+  call lanczos_spectra_isdf(v1, sqrt(sum(v1**2)), ncv, zz(:, 1), nz, niter, isdf_in, wfn, cmtrx, sqrtr1, &
+                            poly, npoly, spec1)
+  call lanczos_spectra_isdf(veig, sqrt(sum(veig**2)), ncv, zz(:, 1), nz, niter, isdf_in, wfn, cmtrx, sqrtr1, &
+                            poly, npoly, speig)
   call ngw_release()
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u) hvec, spectra
+  write (u) hvec, spectra, h1, p1, spec1, speig
   close (u)
 end program ngw_dropin_check

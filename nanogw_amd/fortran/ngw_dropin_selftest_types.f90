@@ -1,11 +1,9 @@
 ! Stand-in types for the compile-and-call check of ngw_dropin.F90 outside NanoGW.
 !
 ! ngw_dropin_selftest_types carries only the components of NanoGW's
-! type(ISDF), type(wavefunction) and w_grp that the drop-in routines read,
-! under the same names.  The program reads a small problem from a stream file
-! (argument 1), calls the drop-in matvec_isdf_BLOCK and
-! lanczos_spectra_isdf_BLOCK exactly as Zcalculate_sigma_lanczos_BLOCK does,
-! and writes Hvec and the spectra to a stream file (argument 2).
+! type(ISDF), type(wavefunction), w_grp and peinf that the drop-in routines
+! read, under the same names (ngw_dropin_check.f90 is the program that uses
+! them).
 module ngw_dropin_selftest_types
   implicit none
   integer, parameter :: dp = kind(1.0d0)
@@ -13,6 +11,7 @@ module ngw_dropin_selftest_types
     integer :: n_intp_r
     real(dp), pointer :: Psi_intp_loc(:, :, :, :)
     real(dp), pointer :: Mmtrx_loc(:, :, :, :, :, :, :)
+    real(dp), pointer :: Mmtrx(:, :, :, :, :, :, :)
   end type ISDF
   type wavefunction
     real(dp), pointer :: e1(:)
@@ -20,6 +19,6 @@ module ngw_dropin_selftest_types
   type grp_t
     integer :: npes = 1
   end type grp_t
-  type(grp_t) :: w_grp
+  type(grp_t) :: w_grp, peinf
 end module ngw_dropin_selftest_types
 

@@ -60,7 +60,7 @@ class IsdfContext:
         h = C.c_void_p()
         _lib.check(self._lib.ngw_create(C.byref(h), int(device)), "ngw_create")
         self._h = h
-        self.nval = self.ncond = self.n_intp = 0
+        self.nval = self.ncond = self.n_intp = self._ncv = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -92,6 +92,20 @@ class IsdfContext:
         self.n_intp, self.nval, self.ncond = n_intp, int(nval), nband - int(nval)
         self.nband = nband
         self.e1 = e1.copy()
+        self._ncv = self.nval * self.ncond
+
+    def set_explicit_c(self, cmtrx, mmtrx, sqrt_r):
+        """The operator of matvec_isdf (shared/apply_matvec.f90:25): the pair
+        matrix C (n_intp, ncv), Mmtrx and sqrtR given explicitly."""
+        cmtrx, mmtrx, sqrt_r = _f64(cmtrx), _f64(mmtrx), _f64(sqrt_r)
+        n_intp, ncv = cmtrx.shape
+        if sqrt_r.shape != (ncv,) or mmtrx.shape != (n_intp, n_intp):
+            raise ValueError("C is (n_intp, ncv), Mmtrx (n_intp, n_intp), sqrtR (ncv,)")
+        _lib.check(self._lib.ngw_set_explicit_c(self._h, n_intp, ncv, _p(cmtrx), n_intp, _p(mmtrx), n_intp,
+                                                _p(sqrt_r)), "ngw_set_explicit_c")
+        self.n_intp, self._ncv = n_intp, ncv
+        self.nval = self.ncond = self.nband = 0
+        self.e1 = None
 
     def set_precision(self, precision):
         p = PREC[precision] if isinstance(precision, str) else int(precision)
@@ -99,7 +113,7 @@ class IsdfContext:
 
     @property
     def ncv(self):
-        return self.nval * self.ncond
+        return self._ncv
 
     def _vecs(self, vec):
         v = _f64(np.asarray(vec, np.float64).reshape(self.ncv, -1, order="F"))

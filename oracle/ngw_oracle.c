@@ -26,6 +26,14 @@
  *   orc_spline_evaluate / orc_newtonr / orc_qp_solve : exactness on cubic
  *                               tables away from the ends, one-step Newton on
  *                               linear f(E), hand-worked spline cases.
+ *   orc_matvec_explicit / orc_polynomial_matvec_explicit : with Cmtrx
+ *                               built from the orbital pair products they
+ *                               equal the BLOCK restatements column by
+ *                               column; a permuted pair list permutes Hvec.
+ *   orc_lanczos_spectra_explicit : same a, b, spectra as the BLOCK
+ *                               restatement when no b_k falls below 1e-8;
+ *                               an eigenvector start breaks down at k = 2
+ *                               and gives 1/(z - lambda) exactly.
  *   orc_polyfit_sqrt          : numpy.linalg.lstsq on the same nodes for
  *                               well-conditioned degrees; at npoly = 12 the
  *                               LAPACK dgesvx coefficients themselves are
@@ -298,6 +306,109 @@ void orc_lanczos_dense(const double *h, long n, const double *v0, double v0_norm
     struct dense_ctx d = {h, n};
     lanczos_block_generic(dense_apply, &d, v0, &v0_norm, n, zz, nz, niter, spectra, 1,
                           a_out, b_out);
+}
+
+/* ------------------------------------------------------------------ */
+/* Single-vector routines on a caller-built Cmtrx                       */
+/* ------------------------------------------------------------------ */
+
+/*
+ * matvec_isdf, shared/apply_matvec.f90:25-140: tmpCmtrx (nmu x ncv, any
+ * pair list) and sqrtR are arguments; one vector.
+ *   Hvec = sqrtR vec                          (:52)
+ *   t = Cmtrx Hvec, u = Mmtrx t, w = Cmtrx^T u  (dgemv :95, :111, :127)
+ *   Hvec = sqrtR^3 Hvec + 4 sqrtR w           (:132, :136)
+ */
+void orc_matvec_explicit(int nmu, long ncv, const double *cm, long ldc, const double *mmat,
+                         long ldm, const double *sqrtR, const double *vec, double *hvec)
+{
+    double *hs = (double *)malloc(sizeof(double) * ncv);
+    double *t = (double *)malloc(sizeof(double) * nmu);
+    double *u = (double *)malloc(sizeof(double) * nmu);
+    double *w = (double *)malloc(sizeof(double) * ncv);
+    for (long i = 0; i < ncv; ++i) hs[i] = sqrtR[i] * vec[i];
+    gemm_nn(nmu, 1, (int)ncv, 1.0, cm, ldc, hs, ncv, 0.0, t, nmu);
+    gemm_nn(nmu, 1, nmu, 1.0, mmat, ldm, t, nmu, 0.0, u, nmu);
+    gemm_tn((int)ncv, 1, nmu, 1.0, cm, ldc, u, nmu, 0.0, w, ncv);
+    for (long i = 0; i < ncv; ++i) {
+        double s = sqrtR[i];
+        double h = s * s * s * hs[i];
+        hvec[i] = h + 4.0 * s * w[i];
+    }
+    free(hs); free(t); free(u); free(w);
+}
+
+/* polynomial_matvec_isdf, shared/apply_matvec.f90:142-186 (Horner, a_n first). */
+void orc_polynomial_matvec_explicit(int nmu, long ncv, const double *cm, long ldc,
+                                    const double *mmat, long ldm, const double *sqrtR,
+                                    const double *vec, double *hvec, const double *poly, int npoly)
+{
+    double *hold = (double *)malloc(sizeof(double) * ncv);
+    for (int k = 1; k <= npoly; ++k) {
+        if (k == 1) {
+            orc_matvec_explicit(nmu, ncv, cm, ldc, mmat, ldm, sqrtR, vec, hvec);
+            for (long i = 0; i < ncv; ++i) hvec[i] = poly[0] * hvec[i] + poly[1] * vec[i]; /* :174 */
+        } else {
+            orc_matvec_explicit(nmu, ncv, cm, ldc, mmat, ldm, sqrtR, hold, hvec);
+            for (long i = 0; i < ncv; ++i) hvec[i] = hvec[i] + poly[k] * vec[i]; /* :179 */
+        }
+        memcpy(hold, hvec, sizeof(double) * ncv); /* :183 */
+    }
+    free(hold);
+}
+
+/*
+ * lanczos_spectra_isdf, shared/lanczos_spectra.f90:7-101, on the explicit
+ * operator.  Unlike the BLOCK variant it leaves the recurrence at the first
+ * |b_k| < 1e-8 (:58).  The reference then runs the continued fraction over
+ * all niter levels with a(k..niter), b(k+1..niter) never set; here the
+ * fraction ends at level k - 1, its exact-arithmetic value (b_k = 0 cuts
+ * every deeper level off).  *nk receives the number of levels used;
+ * coefficients never set are returned as 0.  niter = 1 skips b(2), which the
+ * reference writes past the end of b_elmt(1) (:56).
+ */
+void orc_lanczos_spectra_explicit(int nmu, long ncv, const double *cm, long ldc,
+                                  const double *mmat, long ldm, const double *sqrtR,
+                                  const double *v0, double v0_norm, const double complex *zz, int nz,
+                                  int niter, const double *poly, int npoly,
+                                  double complex *spectra, double *a_out, double *b_out, int *nk)
+{
+    const double eps = 1.0e-8;
+    double *vec = (double *)malloc(sizeof(double) * ncv);
+    double *wvec = (double *)malloc(sizeof(double) * ncv);
+    double *tmp = (double *)malloc(sizeof(double) * ncv);
+    double *a = (double *)calloc((size_t)niter, sizeof(double));
+    double *b = (double *)calloc((size_t)niter, sizeof(double));
+    for (long i = 0; i < ncv; ++i) vec[i] = v0[i] / v0_norm; /* :30 */
+    b[0] = 1.0;                                               /* :35 */
+    orc_polynomial_matvec_explicit(nmu, ncv, cm, ldc, mmat, ldm, sqrtR, vec, wvec, poly, npoly);
+    a[0] = ddot(ncv, wvec, vec); /* :44 */
+    for (long i = 0; i < ncv; ++i) wvec[i] += -a[0] * vec[i];
+    if (niter >= 2) b[1] = sqrt(ddot(ncv, wvec, wvec)); /* :52-56 */
+    int levels = niter;
+    for (int k = 2; k <= niter; ++k) {
+        if (fabs(b[k - 1]) < eps) { /* :58 */
+            levels = k - 1;
+            break;
+        }
+        for (long i = 0; i < ncv; ++i) {
+            wvec[i] = wvec[i] / b[k - 1];
+            vec[i] = -vec[i] * b[k - 1];
+        }
+        double *t = vec; vec = wvec; wvec = t; /* :62-64 */
+        orc_polynomial_matvec_explicit(nmu, ncv, cm, ldc, mmat, ldm, sqrtR, vec, tmp, poly, npoly);
+        for (long i = 0; i < ncv; ++i) wvec[i] += tmp[i];
+        a[k - 1] = ddot(ncv, wvec, vec); /* :69 */
+        if (k + 1 <= niter) {
+            for (long i = 0; i < ncv; ++i) wvec[i] += -a[k - 1] * vec[i];
+            b[k] = sqrt(ddot(ncv, wvec, wvec)); /* :73-77 */
+        }
+    }
+    orc_continued_fraction(levels, a, b, nz, zz, spectra); /* :85-95 */
+    if (a_out) memcpy(a_out, a, sizeof(double) * niter);
+    if (b_out) memcpy(b_out, b, sizeof(double) * niter);
+    if (nk) *nk = levels;
+    free(vec); free(wvec); free(tmp); free(a); free(b);
 }
 
 struct isdf_ctx {

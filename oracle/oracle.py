@@ -63,6 +63,12 @@ def lib():
         L.orc_continued_fraction.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
         L.orc_lanczos_dense.argtypes = [_dp, C.c_long, _dp, C.c_double, _dp, C.c_int, C.c_int,
                                         _dp, _dp, _dp]
+        L.orc_matvec_explicit.argtypes = [C.c_int, C.c_long, _dp, C.c_long, _dp, C.c_long, _dp, _dp, _dp]
+        L.orc_polynomial_matvec_explicit.argtypes = [C.c_int, C.c_long, _dp, C.c_long, _dp, C.c_long, _dp,
+                                                     _dp, _dp, _dp, C.c_int]
+        L.orc_lanczos_spectra_explicit.argtypes = [C.c_int, C.c_long, _dp, C.c_long, _dp, C.c_long, _dp,
+                                                   _dp, C.c_double, _dp, C.c_int, C.c_int, _dp, C.c_int,
+                                                   _dp, _dp, _dp, _ip]
         L.orc_lanczos_spectra_block.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp,
                                                 C.c_long, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
                                                 _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
@@ -164,6 +170,46 @@ def lanczos_spectra_block(psi, mmat, sqrtR, v0, v0_norm, zz, niter, poly, nv, nc
                                     len(poly) - 1, _p(spec), blksz, int(tda), _p(a), _p(b),
                                     int(rebuild))
     return spec, a, b
+
+
+def matvec_explicit(cm, mmat, sqrtR, vec):
+    """matvec_isdf on a caller-built Cmtrx (nmu, ncv): one vector."""
+    cm, mmat, sqrtR, vec = _f64(cm), _f64(mmat), _f64(sqrtR), _f64(vec)
+    nmu, ncv = cm.shape
+    out = np.zeros(ncv)
+    lib().orc_matvec_explicit(nmu, ncv, _p(cm), nmu, _p(mmat), mmat.shape[0], _p(sqrtR), _p(vec), _p(out))
+    return out
+
+
+def polynomial_matvec_explicit(cm, mmat, sqrtR, vec, poly):
+    cm, mmat, sqrtR, vec, poly = _f64(cm), _f64(mmat), _f64(sqrtR), _f64(vec), _f64(poly)
+    nmu, ncv = cm.shape
+    out = np.zeros(ncv)
+    lib().orc_polynomial_matvec_explicit(nmu, ncv, _p(cm), nmu, _p(mmat), mmat.shape[0], _p(sqrtR), _p(vec),
+                                         _p(out), _p(poly), len(poly) - 1)
+    return out
+
+
+def lanczos_spectra_explicit(cm, mmat, sqrtR, v0, v0_norm, zz, niter, poly):
+    """lanczos_spectra_isdf (single vector, breakdown exit).  Returns
+    spectra (nz,), a, b (niter,), and the number of fraction levels used."""
+    cm, mmat, sqrtR, v0, poly = _f64(cm), _f64(mmat), _f64(sqrtR), _f64(v0), _f64(poly)
+    zz = _c128(np.asarray(zz).reshape(-1))
+    nmu, ncv = cm.shape
+    spec = np.zeros(len(zz), np.complex128)
+    a, b = np.zeros(niter), np.zeros(niter)
+    nk = C.c_int(0)
+    lib().orc_lanczos_spectra_explicit(nmu, ncv, _p(cm), nmu, _p(mmat), mmat.shape[0], _p(sqrtR), _p(v0),
+                                       float(v0_norm), _p(zz), len(zz), int(niter), _p(poly), len(poly) - 1,
+                                       _p(spec), _p(a), _p(b), C.byref(nk))
+    return spec, a, b, nk.value
+
+
+def pair_cmtrx(psi, nv, nc):
+    """Cmtrx(mu, icv) = psi(mu, c) psi(mu, v) in the pair order icv = iv + nv*ic."""
+    psi = np.asarray(psi, np.float64)
+    nmu = psi.shape[0]
+    return np.asfortranarray((psi[:, nv:nv + nc][:, :, None] * psi[:, :nv][:, None, :]).reshape(nmu, nc * nv))
 
 
 def pvectors(psi, mmat, sqrtR, m, n0, blksz, nv, nc, tda=False):

@@ -1,6 +1,8 @@
 """The Fortran drop-in (nanogw_amd/fortran/ngw_dropin.F90, NanoGW's
-matvec_isdf_BLOCK / lanczos_spectra_isdf_BLOCK names and argument lists over
-iso_c_binding) called from a Fortran program, against the CPU oracle."""
+matvec_isdf_BLOCK / lanczos_spectra_isdf_BLOCK and single-vector
+matvec_isdf / polynomial_matvec_isdf / lanczos_spectra_isdf names and
+argument lists over iso_c_binding) called from a Fortran program, against the
+CPU oracle."""
 import os
 import subprocess
 
@@ -27,20 +29,44 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     vec = rng.normal(size=(ncv, blksz))
     zz = rng.uniform(-1, 3, size=(17, blksz)) + 0.03j
     zz[-1, :] = 0.0
+    s = O.sqrt_r(m.e1, nv, nc)
+    # an eigenvector of H in the check program's reversed pair order: the
+    # single-vector Lanczos breaks down at k = 2, S(z) = 1/(z - p(lambda))
+    hd = O.matvec_block(m.psi_intp, m.mmtrx, s, np.eye(ncv), nv, nc)
+    w, Z = np.linalg.eigh(0.5 * (hd + hd.T))
+    veig = Z[::-1, 5].copy()
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
         np.array([nmu, nv, nc, blksz, zz.shape[0], niter, len(poly) - 1], np.int32).tofile(f)
         for a in (m.psi_intp, m.mmtrx, m.e1, vec, poly):
             np.asfortranarray(a, np.float64).T.tofile(f)  # column-major bytes
         np.asfortranarray(zz, np.complex128).T.tofile(f)
+        veig.tofile(f)
     r = subprocess.run([DROPIN_CHECK, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(fout, np.float64)
-    hvec = raw[: ncv * blksz].reshape(blksz, ncv).T
-    spec = raw[ncv * blksz:].view(np.complex128).reshape(blksz, zz.shape[0]).T
-    s = O.sqrt_r(m.e1, nv, nc)
+    nz = zz.shape[0]
+    o1 = ncv * blksz
+    o2 = o1 + 2 * nz * blksz
+    hvec = raw[:o1].reshape(blksz, ncv).T
+    spec = raw[o1:o2].view(np.complex128).reshape(blksz, nz).T
+    h1, p1 = raw[o2:o2 + ncv], raw[o2 + ncv:o2 + 2 * ncv]
+    spec1 = raw[o2 + 2 * ncv:o2 + 2 * ncv + 2 * nz].view(np.complex128)
+    speig = raw[o2 + 2 * ncv + 2 * nz:].view(np.complex128)
     ref_h = O.matvec_block(m.psi_intp, m.mmtrx, s, vec, nv, nc)
     ref_s, _, _ = O.lanczos_spectra_block(m.psi_intp, m.mmtrx, s, vec, np.linalg.norm(vec, axis=0), zz, niter,
                                           poly, nv, nc)
     assert rel_err(hvec, ref_h) < 1e-11
     assert rel_err(spec, ref_s) < 1e-6
+    # single-vector routines on tmp_Cmtrx over the reversed pair list
+    cm = np.asfortranarray(O.pair_cmtrx(m.psi_intp, nv, nc)[:, ::-1])
+    sr, v1 = s[::-1].copy(), vec[::-1, 0].copy()
+    assert rel_err(h1, O.matvec_explicit(cm, m.mmtrx, sr, v1)) < 1e-11
+    assert rel_err(p1, O.polynomial_matvec_explicit(cm, m.mmtrx, sr, v1, poly)) < 1e-10
+    ref1, _, _, nk = O.lanczos_spectra_explicit(cm, m.mmtrx, sr, v1, np.linalg.norm(v1), zz[:, 0], niter, poly)
+    assert nk == niter
+    assert rel_err(spec1, ref1) < 1e-6
+    lam = np.polyval(poly, w[5])
+    print("single-vector rel err", rel_err(h1, O.matvec_explicit(cm, m.mmtrx, sr, v1)), rel_err(spec1, ref1),
+          "breakdown", rel_err(speig, 1.0 / (zz[:, 0] - lam)))
+    assert rel_err(speig, 1.0 / (zz[:, 0] - lam)) < 1e-9

@@ -269,3 +269,46 @@ def test_cvt_lattice_lloyd_step_by_brute_force(O):
             assert abs(dist[got[c]] - best) <= 1e-9 * max(1.0, best), (k, c)
             taken[got[c]] = True
         assert len(set(got.tolist())) == n_intp
+
+
+# ------------------------------------------- single-vector explicit-C routines
+def test_explicit_matvec_equals_block_and_permutes(O, tiny):
+    """matvec_isdf on a caller-built Cmtrx equals the BLOCK operator on the
+    orbital pair products, and a permuted pair list permutes Hvec."""
+    H, s, _ = _dense_h(tiny)
+    nv, nc = tiny.nv, tiny.nc
+    cm = O.pair_cmtrx(tiny.psi_intp, nv, nc)
+    x = np.random.default_rng(21).normal(size=nv * nc)
+    y = O.matvec_explicit(cm, tiny.mmtrx, s, x)
+    assert rel_err(y, H @ x) < 1e-9
+    perm = np.random.default_rng(22).permutation(nv * nc)
+    yp = O.matvec_explicit(np.asfortranarray(cm[:, perm]), tiny.mmtrx, s[perm], x[perm])
+    assert rel_err(yp, y[perm]) < 1e-12
+    poly = np.array([0.01, -0.2, 1.3, 0.4])
+    w, Z = np.linalg.eigh(H)
+    yh = O.polynomial_matvec_explicit(cm, tiny.mmtrx, s, x, poly)
+    assert rel_err(yh, Z @ (np.polyval(poly, w) * (Z.T @ x))) < 1e-9
+
+
+def test_explicit_lanczos_equals_block_and_breakdown(O, tiny):
+    """Without breakdown the single-vector Lanczos equals the BLOCK one; from
+    an eigenvector of p(H) it stops at k = 2 and S(z) = 1/(z - p(lambda))."""
+    H, s, _ = _dense_h(tiny)
+    nv, nc = tiny.nv, tiny.nc
+    cm = O.pair_cmtrx(tiny.psi_intp, nv, nc)
+    poly = np.array([0.02, 0.9, 0.1])
+    v = np.random.default_rng(23).normal(size=nv * nc)
+    zz = np.linspace(-1.0, 4.0, 9) + 0.05j
+    spec, a, b, nk = O.lanczos_spectra_explicit(cm, tiny.mmtrx, s, v, np.linalg.norm(v), zz, 6, poly)
+    sref, aref, bref = O.lanczos_spectra_block(tiny.psi_intp, tiny.mmtrx, s, v, [np.linalg.norm(v)], zz, 6, poly,
+                                               nv, nc)
+    assert nk == 6
+    assert rel_err(a, aref[:, 0]) < 1e-12 and rel_err(b, bref[:, 0]) < 1e-12
+    assert rel_err(spec, sref[:, 0]) < 1e-12
+    w, Z = np.linalg.eigh(H)
+    e = Z[:, 3]
+    lam = np.polyval(poly, w[3])
+    spec, a, b, nk = O.lanczos_spectra_explicit(cm, tiny.mmtrx, s, 2.0 * e, 2.0, zz, 6, poly)
+    assert nk == 1 and abs(b[1]) < 1e-8 and np.all(a[1:] == 0.0)
+    assert abs(a[0] - lam) < 1e-10 * abs(lam)
+    assert rel_err(spec, 1.0 / (zz - lam)) < 1e-10
