@@ -103,7 +103,8 @@ int ngw_set_explicit_c(ngw_context* ctx, int n_intp, int ncv, const double* cmtr
  * matvec_isdf_lowcomsym (shared/apply_matvec_lowcomsym.F90:31, :396-403):
  *   full: hvec = R^2 vec + 4 R^1/2 C^T M C R^1/2 vec
  *   TDA : hvec = R vec + 2 C^T M C vec
- * vec, hvec: (ncv, blksz).  ncv must equal nval*ncond.
+ * vec, hvec: (ncv, blksz).  ncv must equal nval*ncond (the columns of C
+ * after ngw_set_explicit_c).
  */
 int ngw_matvec_isdf_block(ngw_context* ctx, const double* vec, double* hvec, int ncv,
                           int blksz, int tamm_d);
