@@ -140,30 +140,33 @@ def fp64_sample(ctx, mol, poly, niter, steps, chunk=200):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c3.md")
 
 
-def measured_traffic(kernel, precision):
+def measured_traffic(kernel, precision, cfg):
     """HBM bytes per launch of a GEMM kernel at the served c3 launch shape
-    (3200 vectors), from the copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE
-    summary committed under profiles/ (lines 'traffic <kernel> <precision>
-    <bytes>'); None when not measured."""
+    (3200 vectors) in tile configuration `cfg`, from the copy-calibrated
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary committed under profiles/
+    (lines 'traffic <kernel> <precision> cfg<index> <bytes>'); None when not
+    measured."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 p = line.split()
-                if len(p) == 4 and p[0] == "traffic" and p[1] == kernel and p[2] == precision:
-                    return float(p[3])
+                if (len(p) == 5 and p[0] == "traffic" and p[1] == kernel and p[2] == precision
+                        and p[3] == f"cfg{cfg}"):
+                    return float(p[4])
     except OSError:
         pass
     return None
 
 
-def roofline_from_profile(prof, flops, precision, traffic_shape_ok=False):
+def roofline_from_profile(prof, flops, precision, traffic_shape_ok=False, tiles=None):
     dom = max(flops, key=lambda k: prof[k][0])
     ms, cnt = prof[dom]
     if cnt == 0 or ms <= 0:
         return None
     ach = flops[dom] / (ms / 1000.0)
     peak = PEAK[precision]
-    traffic = measured_traffic(dom, precision) if traffic_shape_ok else None
+    cfg = None if tiles is None else {"K1": tiles[0], "K3": tiles[1]}.get(dom)
+    traffic = measured_traffic(dom, precision, cfg) if traffic_shape_ok and cfg is not None else None
     return {"bound": "mfma", "kernel": dom, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": ach / peak, "traffic": traffic, "avg_launch_us": ms * 1000.0 / cnt, "launches": cnt,
             "flop_per_launch": flops[dom] / cnt}
@@ -464,7 +467,7 @@ def main():
     # the committed PMC traffic was measured at the launch shape of the default
     # served c3 run: both timed states (2 x 1600 vectors) in every launch
     shape_ok = args.config == "c3" and args.mode == "served" and args.steps == 2
-    roof = roofline_from_profile(prof, flops, args.precision, shape_ok) if prof["K1"][1] else None
+    roof = roofline_from_profile(prof, flops, args.precision, shape_ok, tiles) if prof["K1"][1] else None
     f_mv = 4.0 * nmu * nv * nc + 2.0 * nmu * nmu
 
     # QP error against the stored FP64 reference of this molecule

@@ -68,7 +68,7 @@ constexpr LpCfg K1_CFG[3][LP_NCFG] = {
 constexpr LpCfg K3_CFG[3][LP_NCFG] = {
     {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {128, 128, 2, 8, 1, 1}},
     {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {64, 128, 2, 8}},
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 8, 0, 1}, {128, 128, 3, 4}}};  // FP16: W from wprep
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {192, 128, 3, 8, 0, 1}, {128, 192, 2, 4, 0, 1}}};  // FP16: W from wprep
 constexpr int NXCD = 8;
 __host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : kind == LP_F16 ? 2 : 0; }
 static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : prec == PREC_F16 ? 2 : 0; }
