@@ -1,11 +1,12 @@
 ! Drop-in replacements of NanoGW's Lanczos hot-path routines on libngw.
 !
 ! Link this file into NanoGW instead of shared/apply_matvec_BLOCK.F90,
-! shared/lanczos_spectra_BLOCK.F90, shared/apply_matvec.f90 and
-! shared/lanczos_spectra.f90: the external subroutines keep the reference's
-! names and argument lists (implicit interfaces, arguments by reference), so
-! Zcalculate_sigma_lanczos_BLOCK and Zcalculate_sigma_lanczos call them
-! unchanged.
+! shared/lanczos_spectra_BLOCK.F90, shared/apply_matvec.f90,
+! shared/lanczos_spectra.f90, shared/apply_matvec_UltraLowMem.F90 and
+! shared/lanczos_spectra_UltraLowMem.F90: the external subroutines keep the
+! reference's names and argument lists (implicit interfaces, arguments by
+! reference), so Zcalculate_sigma_lanczos_BLOCK, Zcalculate_sigma_lanczos and
+! Zcalculate_sigma_lanczos_UltraLowMem call them unchanged.
 !
 ! BLOCK routines: the operator (Psi at the ISDF points, Mmtrx, energies) is
 ! uploaded to the GPU once and kept resident; it is re-uploaded when the
@@ -14,6 +15,10 @@
 ! in NanoGW's conduction-outer / valence-inner order
 ! (shared/setup_g.F90z:154-168); other layouts stop with a message, as
 ! NanoGW's die() would.
+!
+! UltraLowMem routines (matvec_isdf_UltraLowMem and its polynomial and
+! Lanczos companions): single vectors on the BLOCK routines' resident
+! operator, same scope.
 !
 ! Single-vector routines (matvec_isdf, polynomial_matvec_isdf,
 ! lanczos_spectra_isdf): the caller's tmpCmtrx, Mmtrx and sqrtR are uploaded
@@ -143,6 +148,57 @@ contains
     up_cnintp = -1
   end subroutine ngw_ensure
 
+  ! The single-vector reference Lanczos routines leave the recurrence at the
+  ! first |b_k| < 1e-8 and then read a(k..niter), b(k+1..niter) unset in
+  ! the continued fraction.  Given the coefficients of the full recurrence,
+  ! this ends the fraction at level k - 1 instead, its exact-arithmetic value;
+  ! spectra are left as they are when no b_k is that small.
+  subroutine ngw_breakdown_fraction(a_elmt, b_elmt, niter, zz, nz, spectra)
+    integer, intent(in) :: niter, nz
+    real(c_double), intent(in) :: a_elmt(niter), b_elmt(niter)
+    complex(c_double_complex), intent(in) :: zz(nz)
+    complex(c_double_complex), intent(inout) :: spectra(nz)
+    real(c_double), parameter :: eps = 1.0d-8
+    complex(c_double_complex) :: s, enum
+    integer :: k, j, iz, levels
+    levels = niter
+    do k = 2, niter
+      if (abs(b_elmt(k)) < eps) then
+        levels = k - 1
+        exit
+      end if
+    end do
+    if (levels == niter) return
+    do iz = 1, nz
+      s = (0.d0, 0.d0)
+      do j = levels, 1, -1
+        enum = b_elmt(j) * b_elmt(j)
+        if (j == 1) enum = (1.d0, 0.d0)
+        s = enum / (zz(iz) - a_elmt(j) - s)
+      end do
+      spectra(iz) = s
+    end do
+  end subroutine ngw_breakdown_fraction
+
+  ! Single-vector Lanczos on the resident operator with the reference's
+  ! breakdown exit applied to the fraction.
+  subroutine ngw_lanczos_single(v0, v0_norm, ncv, zz, nz, niter, polynomial, npoly, spectra, what)
+    integer, intent(in) :: ncv, nz, niter, npoly
+    real(c_double), intent(in) :: v0(ncv), v0_norm, polynomial(npoly + 1)
+    complex(c_double_complex), intent(in) :: zz(nz)
+    complex(c_double_complex), intent(out) :: spectra(nz)
+    character(len=*), intent(in) :: what
+    real(c_double), allocatable, target :: a_elmt(:), b_elmt(:)
+    real(c_double) :: vn(1)
+    allocate (a_elmt(niter), b_elmt(niter))
+    vn(1) = v0_norm
+    call ngw_check(ngw_lanczos_spectra_isdf_block(ngw_ctx, v0, vn, int(ncv, c_int), zz, int(nz, c_int), &
+                                                  int(niter, c_int), polynomial, int(npoly, c_int), spectra, &
+                                                  1_c_int, 0_c_int, c_loc(a_elmt), c_loc(b_elmt)), what)
+    call ngw_breakdown_fraction(a_elmt, b_elmt, niter, zz, nz, spectra)
+    deallocate (a_elmt, b_elmt)
+  end subroutine ngw_lanczos_single
+
   subroutine ngw_release()
     integer(c_int) :: st
     if (c_associated(ngw_ctx)) st = ngw_destroy(ngw_ctx)
@@ -267,10 +323,7 @@ subroutine polynomial_matvec_isdf(vec, Hvec, ncv_loc, polynomial, npoly, isdf_in
                                                   int(npoly, c_int), 1_c_int, 0_c_int), 'polynomial_matvec_isdf')
 end subroutine polynomial_matvec_isdf
 
-! shared/lanczos_spectra.f90:7.  The reference leaves the recurrence at the
-! first |b_k| < 1e-8 (:58) and then reads a(k..niter), b(k+1..niter) unset
-! in the continued fraction; here the fraction ends at level k - 1, its
-! exact-arithmetic value.  Without a breakdown the spectra are the library's.
+! shared/lanczos_spectra.f90:7 (breakdown exit :58, see ngw_breakdown_fraction)
 subroutine lanczos_spectra_isdf(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_in, &
                                 wfn, tmp_Cmtrx, sqrtR, polynomial, npoly, spectra)
   NGW_TYPES
@@ -283,37 +336,81 @@ subroutine lanczos_spectra_isdf(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_in, &
                           tmp_Cmtrx(isdf_in%n_intp_r, ncv_loc), sqrtR(ncv_loc)
   complex(dp), intent(out) :: spectra(nz)
   type(wavefunction), intent(in) :: wfn
-  real(c_double), allocatable, target :: a_elmt(:), b_elmt(:)
-  real(c_double) :: vn(1)
-  real(dp), parameter :: eps = 1.0d-8
-  complex(dp) :: s, enum
-  integer :: k, j, iz, levels
   if (peinf%npes /= 1) stop 'libngw drop-in: one rank per GPU (peinf%npes = 1)'
   call ngw_ensure_explicit(isdf_in%n_intp_r, ncv_loc, tmp_Cmtrx, size(isdf_in%Mmtrx, 1), &
                            isdf_in%Mmtrx(:, :, 1, 1, 1, 1, 1), sqrtR)
-  allocate (a_elmt(niter), b_elmt(niter))
-  vn(1) = v0_norm
-  call ngw_check(ngw_lanczos_spectra_isdf_block(ngw_ctx, v0, vn, int(ncv_loc, c_int), zz, int(nz, c_int), &
-                                                int(niter, c_int), polynomial, int(npoly, c_int), spectra, &
-                                                1_c_int, 0_c_int, c_loc(a_elmt), c_loc(b_elmt)), &
-                 'lanczos_spectra_isdf')
-  levels = niter
-  do k = 2, niter
-    if (abs(b_elmt(k)) < eps) then
-      levels = k - 1
-      exit
-    end if
-  end do
-  if (levels < niter) then
-    do iz = 1, nz
-      s = (0.d0, 0.d0)
-      do j = levels, 1, -1
-        enum = b_elmt(j) * b_elmt(j)
-        if (j == 1) enum = (1.d0, 0.d0)
-        s = enum / (zz(iz) - a_elmt(j) - s)
-      end do
-      spectra(iz) = s
-    end do
-  end if
-  deallocate (a_elmt, b_elmt)
+  call ngw_lanczos_single(v0, v0_norm, ncv_loc, zz, nz, niter, polynomial, npoly, spectra, 'lanczos_spectra_isdf')
 end subroutine lanczos_spectra_isdf
+
+! shared/apply_matvec_UltraLowMem.F90:25 (the pair list has two rows)
+subroutine matvec_isdf_UltraLowMem(vec, Hvec, ncv_loc, isdf_in, wfn, cvpair, sqrtR, &
+                                   MC_vec, CMC_vec, tmp_vec, tmp_Hvec, tmp_cvpair)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc
+  real(dp), intent(in) :: vec(ncv_loc), sqrtR(ncv_loc)
+  real(dp), intent(out) :: Hvec(ncv_loc)
+  type(ISDF), intent(in) :: isdf_in
+  type(wavefunction), intent(in) :: wfn
+  integer, intent(in) :: cvpair(2, ncv_loc)
+  real(dp), intent(inout) :: MC_vec(*), CMC_vec(*), tmp_vec(*), tmp_Hvec(*)
+  integer, intent(inout) :: tmp_cvpair(*)
+  integer :: nval, nband
+  if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
+  call ngw_pairs(cvpair, 2, ncv_loc, nval, nband)
+  call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+  call ngw_check(ngw_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), 1_c_int, 0_c_int), &
+                 'matvec_isdf_UltraLowMem')
+end subroutine matvec_isdf_UltraLowMem
+
+! shared/apply_matvec_UltraLowMem.F90:168
+subroutine polynomial_matvec_isdf_UltraLowMem(vec, Hvec, ncv_loc, polynomial, npoly, isdf_in, wfn, cvpair, sqrtR, &
+                                              MC_vec, CMC_vec, tmp_vec, tmp_Hvec, tmp_cvpair)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, npoly
+  real(dp), intent(in) :: vec(ncv_loc), polynomial(npoly + 1), sqrtR(ncv_loc)
+  real(dp), intent(out) :: Hvec(ncv_loc)
+  type(ISDF), intent(in) :: isdf_in
+  type(wavefunction), intent(in) :: wfn
+  integer, intent(in) :: cvpair(2, ncv_loc)
+  real(dp), intent(inout) :: MC_vec(*), CMC_vec(*), tmp_vec(*), tmp_Hvec(*)
+  integer, intent(inout) :: tmp_cvpair(*)
+  integer :: nval, nband
+  if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
+  call ngw_pairs(cvpair, 2, ncv_loc, nval, nband)
+  call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+  call ngw_check(ngw_polynomial_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), polynomial, &
+                                                  int(npoly, c_int), 1_c_int, 0_c_int), &
+                 'polynomial_matvec_isdf_UltraLowMem')
+end subroutine polynomial_matvec_isdf_UltraLowMem
+
+! shared/lanczos_spectra_UltraLowMem.F90:7 (breakdown exit :66, see
+! ngw_breakdown_fraction; cvpair is (2, ncv_loc) as the driver passes it)
+subroutine lanczos_spectra_isdf_UltraLowMem(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_in, &
+                                            wfn, cvpair, sqrtR, polynomial, npoly, spectra, &
+                                            MC_vec, CMC_vec, tmp_vec, tmp_Hvec, tmp_cvpair)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, niter, nz, npoly
+  complex(dp), intent(in) :: zz(nz)
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(in) :: v0(ncv_loc), polynomial(npoly + 1), v0_norm, sqrtR(ncv_loc)
+  integer, intent(in) :: cvpair(2, ncv_loc)
+  complex(dp), intent(out) :: spectra(nz)
+  type(wavefunction), intent(in) :: wfn
+  real(dp), intent(inout) :: MC_vec(*), CMC_vec(*), tmp_vec(*), tmp_Hvec(*)
+  integer, intent(inout) :: tmp_cvpair(*)
+  integer :: nval, nband
+  if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
+  call ngw_pairs(cvpair, 2, ncv_loc, nval, nband)
+  call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+  call ngw_lanczos_single(v0, v0_norm, ncv_loc, zz, nz, niter, polynomial, npoly, spectra, &
+                          'lanczos_spectra_isdf_UltraLowMem')
+end subroutine lanczos_spectra_isdf_UltraLowMem

@@ -4,8 +4,8 @@
 ! lanczos_spectra_isdf_BLOCK as Zcalculate_sigma_lanczos_BLOCK does, then the
 ! single-vector matvec_isdf, polynomial_matvec_isdf and lanczos_spectra_isdf
 ! as Zcalculate_sigma_lanczos does (tmp_Cmtrx built from Psi, here over the
-! pair list in reverse order), and writes the results to a stream file
-! (argument 2).
+! pair list in reverse order), then their UltraLowMem variants, and writes
+! the results to a stream file (argument 2).
 program ngw_dropin_check
   use ngw_dropin_selftest_types
   use ngw_session, only: ngw_release
@@ -18,6 +18,9 @@ program ngw_dropin_check
   real(dp), allocatable :: cmtrx(:, :), sqrtr1(:), v1(:), h1(:), p1(:), veig(:)
   complex(dp), allocatable :: zz(:, :), spectra(:, :), spec1(:), speig(:)
   integer :: iv, ic, jcv
+  integer, allocatable :: cvpair2(:, :)
+  real(dp), allocatable :: hu(:), pu(:), veig2(:)
+  complex(dp), allocatable :: specu(:), speigu(:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -62,8 +65,22 @@ program ngw_dropin_check
                             poly, npoly, spec1)
   call lanczos_spectra_isdf(veig, sqrt(sum(veig**2)), ncv, zz(:, 1), nz, niter, isdf_in, wfn, cmtrx, sqrtr1, &
                             poly, npoly, speig)
+  ! UltraLowMem variants: two-row pair list in NanoGW's order
+  allocate (cvpair2(2, ncv), hu(ncv), pu(ncv), veig2(ncv), specu(nz), speigu(nz))
+  cvpair2 = cvpair(1:2, :)
+  do icv = 1, ncv
+    veig2(icv) = veig(ncv + 1 - icv)
+  end do
+  call matvec_isdf_UltraLowMem(vec(:, 1), hu, ncv, isdf_in, wfn, cvpair2, sqrtR, dummy, dummy, dummy, dummy, &
+                               tmp_cvpair)
+  call polynomial_matvec_isdf_UltraLowMem(vec(:, 1), pu, ncv, poly, npoly, isdf_in, wfn, cvpair2, sqrtR, dummy, &
+                                          dummy, dummy, dummy, tmp_cvpair)
+  call lanczos_spectra_isdf_UltraLowMem(vec(:, 1), v0norm(1), ncv, zz(:, 1), nz, niter, isdf_in, wfn, cvpair2, &
+                                        sqrtR, poly, npoly, specu, dummy, dummy, dummy, dummy, tmp_cvpair)
+  call lanczos_spectra_isdf_UltraLowMem(veig2, sqrt(sum(veig2**2)), ncv, zz(:, 1), nz, niter, isdf_in, wfn, &
+                                        cvpair2, sqrtR, poly, npoly, speigu, dummy, dummy, dummy, dummy, tmp_cvpair)
   call ngw_release()
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u) hvec, spectra, h1, p1, spec1, speig
+  write (u) hvec, spectra, h1, p1, spec1, speig, hu, pu, specu, speigu
   close (u)
 end program ngw_dropin_check

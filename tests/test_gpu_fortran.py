@@ -1,8 +1,8 @@
 """The Fortran drop-in (nanogw_amd/fortran/ngw_dropin.F90, NanoGW's
-matvec_isdf_BLOCK / lanczos_spectra_isdf_BLOCK and single-vector
-matvec_isdf / polynomial_matvec_isdf / lanczos_spectra_isdf names and
-argument lists over iso_c_binding) called from a Fortran program, against the
-CPU oracle."""
+matvec_isdf_BLOCK / lanczos_spectra_isdf_BLOCK, single-vector
+matvec_isdf / polynomial_matvec_isdf / lanczos_spectra_isdf and their
+UltraLowMem variants, with the reference names and argument lists over
+iso_c_binding) called from a Fortran program, against the CPU oracle."""
 import os
 import subprocess
 
@@ -51,8 +51,13 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     hvec = raw[:o1].reshape(blksz, ncv).T
     spec = raw[o1:o2].view(np.complex128).reshape(blksz, nz).T
     h1, p1 = raw[o2:o2 + ncv], raw[o2 + ncv:o2 + 2 * ncv]
+    o3 = o2 + 2 * ncv + 4 * nz
     spec1 = raw[o2 + 2 * ncv:o2 + 2 * ncv + 2 * nz].view(np.complex128)
-    speig = raw[o2 + 2 * ncv + 2 * nz:].view(np.complex128)
+    speig = raw[o2 + 2 * ncv + 2 * nz:o3].view(np.complex128)
+    hu, pu = raw[o3:o3 + ncv], raw[o3 + ncv:o3 + 2 * ncv]
+    specu = raw[o3 + 2 * ncv:o3 + 2 * ncv + 2 * nz].view(np.complex128)
+    speigu = raw[o3 + 2 * ncv + 2 * nz:].view(np.complex128)
+    assert len(speigu) == nz
     ref_h = O.matvec_block(m.psi_intp, m.mmtrx, s, vec, nv, nc)
     ref_s, _, _ = O.lanczos_spectra_block(m.psi_intp, m.mmtrx, s, vec, np.linalg.norm(vec, axis=0), zz, niter,
                                           poly, nv, nc)
@@ -70,3 +75,11 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     print("single-vector rel err", rel_err(h1, O.matvec_explicit(cm, m.mmtrx, sr, v1)), rel_err(spec1, ref1),
           "breakdown", rel_err(speig, 1.0 / (zz[:, 0] - lam)))
     assert rel_err(speig, 1.0 / (zz[:, 0] - lam)) < 1e-9
+    # UltraLowMem variants on the factored operator, NanoGW's pair order
+    v = vec[:, 0]
+    assert rel_err(hu, O.matvec_block(m.psi_intp, m.mmtrx, s, v, nv, nc)[:, 0]) < 1e-11
+    assert rel_err(pu, O.polynomial_matvec_block(m.psi_intp, m.mmtrx, s, v, nv, nc, poly)[:, 0]) < 1e-10
+    refu, _, _ = O.lanczos_spectra_block(m.psi_intp, m.mmtrx, s, v, [np.linalg.norm(v)], zz[:, :1], niter, poly,
+                                         nv, nc)
+    assert rel_err(specu, refu[:, 0]) < 1e-6
+    assert rel_err(speigu, 1.0 / (zz[:, 0] - lam)) < 1e-9
