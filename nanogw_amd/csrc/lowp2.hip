@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, 
     const LpStage& sg = ops ? st[r * nstage + s] : st1;
     const bool x32 = sg.flags & ST_X32;
     const long xoff = (long)b * op.ld;
-    const long chunk = (ncv + split - 1) / split;
+    const long chunk = ((ncv + split - 1) / split + 3) & ~3L;  // a multiple of four elements
     const long lo = part * chunk, hi = min(ncv, lo + chunk);
     double m = 0.0;
     auto scan = [&](auto* xp) {
@@ -274,8 +274,45 @@ __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, 
             for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
         }
     };
-    if (x32) scan(reinterpret_cast<const float*>(sg.x));
-    else scan(sg.x);
+    // FP32 input with 16-byte aligned starts: four elements per load
+    auto scan4 = [&](const float* xp) {
+        const float4* x4 = reinterpret_cast<const float4*>(xp + xoff + lo);
+        const float4* s4 = reinterpret_cast<const float4*>(op.xsig + lo);
+        const long nq = hi > lo ? (hi - lo) >> 2 : 0;
+        for (long q0 = threadIdx.x; q0 < nq; q0 += 256 * 4) {
+            float4 xv[4], sv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long q = q0 + 256 * u;
+                if (q < nq) {
+                    xv[u] = x4[q];
+                    sv[u] = s4[q];
+                } else {
+                    xv[u] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+                    sv[u] = xv[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                m = fmax(m, fabs((double)xv[u].x) * (double)sv[u].x);
+                m = fmax(m, fabs((double)xv[u].y) * (double)sv[u].y);
+                m = fmax(m, fabs((double)xv[u].z) * (double)sv[u].z);
+                m = fmax(m, fabs((double)xv[u].w) * (double)sv[u].w);
+            }
+        }
+        for (long idx = lo + 4 * nq + threadIdx.x; idx < hi; idx += 256)
+            m = fmax(m, fabs((double)xp[xoff + idx]) * (double)op.xsig[idx]);
+    };
+    if (x32) {
+        const float* xf = reinterpret_cast<const float*>(sg.x);
+        if ((((xoff + lo) & 3) == 0) && ((reinterpret_cast<uintptr_t>(xf) & 15) == 0) &&
+            ((reinterpret_cast<uintptr_t>(op.xsig) & 15) == 0))
+            scan4(xf);
+        else
+            scan(xf);
+    } else {
+        scan(sg.x);
+    }
     const float mb = block_max256((float)m);
     if (threadIdx.x == 0) op.xpart[b * XS_SPLIT + part] = mb;
 }
@@ -286,8 +323,8 @@ __global__ __launch_bounds__(256) void xscale_kernel(const LpOp* ops, LpOp op1, 
 // LDS tile CT (VT + 1) fits XP_TILE floats (and CT <= ldc1): 4-8k elements
 // whatever nv is.  For
 // nv <= 128 the block is one contiguous run of x, read flat and fully
-// coalesced; it is transposed through LDS and each thread writes eight
-// consecutive c in one 16- or 32-byte store.
+// coalesced (FP32 input four elements per load); it is transposed through
+// LDS and each thread writes eight consecutive c in one 16- or 32-byte store.
 constexpr int XP_V = 128, XP_TILE = 10240;
 __host__ __device__ inline int xp_vt(int nv) { return nv < XP_V ? nv : XP_V; }
 __host__ __device__ inline int xp_ct(int nv, int ldc1)
@@ -351,8 +388,59 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
                 if (e0 + 256 * u < tot) tile[cc[u] * pitch + vv[u]] = (float)(xv[u] * (sv[u] * f16s));
         }
     };
-    if (x32) walk(reinterpret_cast<const float*>(sg.x));
-    else walk(sg.x);
+    // FP32 input over whole v columns (vn == nv) is one contiguous run of x
+    // and of xsig: four elements per 16-byte load when both starts are
+    // 16-byte aligned; the last tot mod 4 elements go element-wise.
+    auto walk4 = [&](const float* xp) {
+        const float4* x4 = reinterpret_cast<const float4*>(xp + xoff + base);
+        const float4* s4 = reinterpret_cast<const float4*>(op.xsig + base);
+        const int nq = tot >> 2;
+        for (int q0 = threadIdx.x; q0 < nq; q0 += 256 * 4) {
+            float4 xv[4], sv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + 256 * u;
+                if (q < nq) {
+                    xv[u] = x4[q];
+                    sv[u] = s4[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + 256 * u;
+                if (q >= nq) continue;
+                const float xa[4] = {xv[u].x, xv[u].y, xv[u].z, xv[u].w};
+                const float sa[4] = {sv[u].x, sv[u].y, sv[u].z, sv[u].w};
+                const int e = 4 * q;
+                int c = (int)((float)e * inv);
+                if (c * vn > e) --c;
+                if ((c + 1) * vn <= e) ++c;
+                int v = e - c * vn;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    tile[c * pitch + v] = (float)((double)xa[k] * ((double)sa[k] * f16s));
+                    if (++v == vn) {
+                        v = 0;
+                        ++c;
+                    }
+                }
+            }
+        }
+        for (int e = 4 * nq + threadIdx.x; e < tot; e += 256) {
+            const int c = e / vn, v = e - c * vn;
+            tile[c * pitch + v] = (float)((double)xp[xoff + base + e] * ((double)op.xsig[base + e] * f16s));
+        }
+    };
+    if (x32) {
+        const float* xf = reinterpret_cast<const float*>(sg.x);
+        const bool aligned = (((xoff + base) & 3) == 0) && ((base & 3) == 0) &&
+                             ((reinterpret_cast<uintptr_t>(xf) & 15) == 0) &&
+                             ((reinterpret_cast<uintptr_t>(op.xsig) & 15) == 0);
+        if (vn == op.nv && aligned) walk4(xf);
+        else walk(xf);
+    } else {
+        walk(sg.x);
+    }
     for (int i = cn * pitch + threadIdx.x; i < ct * pitch; i += 256) tile[i] = 0.0f;  // k padding rows
     __syncthreads();
     const int cpr = ct / 8;  // 8-wide chunks per row (a power of two)
