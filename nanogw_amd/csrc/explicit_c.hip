@@ -4,10 +4,11 @@
 // tmpCmtrx(n_intp, ncv) with C(mu, icv) = Psi_v(mu) Psi_c(mu), and
 //   y = R^3/2 (R^1/2 x) + 4 R^1/2 C^T M C (R^1/2 x)       (full, :126-131)
 //   y = R x + 2 C^T M C x                                 (TDA)
-// The three products are plain library GEMMs (rocBLAS dgemm: for one vector
-// they are the reference's dgemv calls); the diagonal term and the Horner /
-// Lanczos combination are the same epilogue as the factored operator.
-// FP64 only.
+// The three products are library GEMMs (rocBLAS dgemm) for several vectors;
+// for one vector (the reference's three dgemv calls) C x and M t run on the
+// column-split GEMV kernels below and C^T u on rocBLAS dgemv.  The diagonal
+// term and the Horner / Lanczos combination are the same epilogue as the
+// factored operator.  FP64 only.
 
 #include "ngw_kernels.h"
 
@@ -40,6 +41,73 @@ __global__ void explicit_epilogue_kernel(OperatorEpilogue ep, const double* y, l
         ep.apply(i, b, y[t]);
     }
 }
+
+}  // namespace
+
+// y = A x for a column-major m x n matrix A with m small against n (C x
+// and M t of one vector; M t is also the factored operator's one-vector K2): workgroup (row block rb, column chunk cc) sums its 256 rows over
+// its columns, reading each column segment as one coalesced run; part(cc, i)
+// holds the chunk sums, added in chunk order by gemv_wide_reduce (the
+// result does not depend on scheduling).
+constexpr int GV_MAX_CHUNKS = 1024;
+__global__ __launch_bounds__(256) void gemv_wide_partial(const double* A, long lda, int m, int n, const double* x,
+                                                         int cw, double* part)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int cc = blockIdx.y;
+    const int j0 = cc * cw, j1 = min(n, j0 + cw);
+    if (i >= m) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = j0;
+    for (; j + 3 < j1; j += 4) {
+        s0 += A[i + (long)j * lda] * x[j];
+        s1 += A[i + (long)(j + 1) * lda] * x[j + 1];
+        s2 += A[i + (long)(j + 2) * lda] * x[j + 2];
+        s3 += A[i + (long)(j + 3) * lda] * x[j + 3];
+    }
+    for (; j < j1; ++j) s0 += A[i + (long)j * lda] * x[j];
+    part[(long)cc * m + i] = (s0 + s1) + (s2 + s3);
+}
+
+// 64 rows per workgroup; wave g sums chunks g, g + 4, g + 8, ... (four
+// loads in flight per thread), and the four wave sums are added in LDS.
+__global__ __launch_bounds__(256) void gemv_wide_reduce(const double* part, int m, int nchunk, double* y)
+{
+    __shared__ double red[4][64];
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + r;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (i < m) {
+        int c = g;
+        for (; c + 12 < nchunk; c += 16) {
+            s0 += part[(long)c * m + i];
+            s1 += part[(long)(c + 4) * m + i];
+            s2 += part[(long)(c + 8) * m + i];
+            s3 += part[(long)(c + 12) * m + i];
+        }
+        for (; c < nchunk; c += 4) s0 += part[(long)c * m + i];
+    }
+    red[g][r] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (g == 0 && i < m) y[i] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+}
+
+int gemv_wide(hipStream_t s, const double* A, long lda, int m, int n, const double* x, double* y, DevBuf& part)
+{
+    const int rblk = (m + 255) / 256;
+    const int cap = std::max(1, std::min(GV_MAX_CHUNKS, 2048 / rblk));  // depends on m only: one size per operator
+    int nchunk = std::max(1, std::min(cap, (n + 31) / 32));
+    const int cw = (n + nchunk - 1) / nchunk;
+    nchunk = (n + cw - 1) / cw;
+    NGW_TRY(part.ensure(sizeof(double) * (size_t)cap * m));
+    hipLaunchKernelGGL(gemv_wide_partial, dim3(rblk, nchunk), dim3(256), 0, s, A, lda, m, n, x, cw,
+                       part.as<double>());
+    hipLaunchKernelGGL(gemv_wide_reduce, dim3((m + 63) / 64), dim3(256), 0, s, part.as<const double>(), m, nchunk, y);
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
+}
+
+namespace {
 
 int rb(rocblas_status st, const char* what)
 {
@@ -77,13 +145,23 @@ int operator_explicit(ngw_context* ctx, const OperatorArgs& a)
     NGW_HIP(hipGetLastError());
     const double one = 1.0, zero = 0.0;
     const double* C = ctx->cexp.as<double>();
-    // t = C xs, u = M t, y = C^T u
-    NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nmu, B, (int)ncv, &one, C, nmu, xs,
-                             (int)ncv, &zero, t, nmu), "dgemm C x"));
-    NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nmu, B, nmu, &one,
-                             ctx->mmat.as<double>(), nmu, t, nmu, &zero, u, nmu), "dgemm M t"));
-    NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (int)ncv, B, nmu, &one, C, nmu,
-                             u, nmu, &zero, y, (int)ncv), "dgemm C^T u"));
+    // t = C xs, u = M t, y = C^T u: GEMV for one vector (the library's GEMM
+    // kernels tile n = 1 as a full 128-wide block; its plain GEMV splits only
+    // the n_intp rows of the wide C x over workgroups), GEMM otherwise
+    const double* M = ctx->mmat.as<double>();
+    if (B == 1) {
+        NGW_TRY(gemv_wide(s, C, nmu, nmu, (int)ncv, xs, t, ctx->gv_part));
+        NGW_TRY(gemv_wide(s, M, nmu, nmu, nmu, t, u, ctx->gv_part));
+        NGW_TRY(rb(rocblas_dgemv(h, rocblas_operation_transpose, nmu, (int)ncv, &one, C, nmu, u, 1, &zero, y, 1),
+                   "dgemv C^T u"));
+    } else {
+        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nmu, B, (int)ncv, &one, C, nmu,
+                                 xs, (int)ncv, &zero, t, nmu), "dgemm C x"));
+        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nmu, B, nmu, &one, M, nmu, t,
+                                 nmu, &zero, u, nmu), "dgemm M t"));
+        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (int)ncv, B, nmu, &one, C,
+                                 nmu, u, nmu, &zero, y, (int)ncv), "dgemm C^T u"));
+    }
     hipLaunchKernelGGL(explicit_epilogue_kernel, dim3(2048), dim3(256), 0, s, a.epilogue(ctx), y, ncv, B);
     NGW_HIP(hipGetLastError());
     return NGW_OK;

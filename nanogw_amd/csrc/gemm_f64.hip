@@ -215,7 +215,10 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a)
     NGW_TRY(launch_k1(ctx, a.x, a.ldx, sx, B, tpart, t));
     {
         ProfScope ps(ctx, KC_K2);
-        NGW_TRY(launch_k2(ctx, ColMajorB{t, (long)nmu, B}, B, u));
+        if (B == 1)  // one vector: u = M t is a GEMV
+            NGW_TRY(gemv_wide(ctx->stream, ctx->mmat.as<double>(), ctx->ld_m, nmu, nmu, t, u, ctx->gv_part));
+        else
+            NGW_TRY(launch_k2(ctx, ColMajorB{t, (long)nmu, B}, B, u));
     }
     {
         ProfScope ps(ctx, KC_K3);

@@ -94,6 +94,7 @@ struct ngw_context {
     // the factored Psi_v o Psi_c; its rocBLAS handle and GEMM workspaces
     bool explicit_c = false;
     ngw::DevBuf cexp, ex_x, ex_y, ex_t;
+    ngw::DevBuf gv_part;  // chunk sums of the one-vector GEMV (gemv_wide)
     void* rb_handle = nullptr;
     // workspaces
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;

@@ -120,6 +120,9 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a);
 int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int nvec, int tda,
                  double* d_p, long ldp_out);
 
+// y = A x for a column-major m x n A, split over column chunks, summed in a
+// fixed order; part is the chunk-sum workspace.
+int gemv_wide(hipStream_t s, const double* A, long lda, int m, int n, const double* x, double* y, DevBuf& part);
 // The operator with an explicit pair matrix C (rocBLAS products, FP64).
 int operator_explicit(ngw_context* ctx, const OperatorArgs& a);
 void explicit_release(ngw_context* ctx);

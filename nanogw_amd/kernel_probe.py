@@ -10,6 +10,12 @@ plain device-to-device copy of --copy-bytes (the reference kernel that
 converts a hardware byte counter into a scale factor).  Kernel timings and counters come from the profiler
 wrapped around this command; the script prints what it launched.
 
+With --single-vector the probe instead times one-vector FP64 applications
+(the shape of NanoGW's matvec_isdf / lanczos_spectra_isdf calls) in both
+operator forms: the factored one (ngw_set_isdf) and the explicit pair matrix
+C (ngw_set_explicit_c), --reps applications each, and prints the time per
+application against the bytes each form has to read.
+
   python -m nanogw_amd.kernel_probe --nmu 5000 --nv 100 --nc 1500 \\
       --nvec 1600 --precision bf16 --reps 3
 """
@@ -32,6 +38,8 @@ def main(argv=None):
     ap.add_argument("--tda", action="store_true")
     ap.add_argument("--lanczos", type=int, default=0,
                     help="run this many Lanczos steps (12 operator applications each) instead of --reps matvecs")
+    ap.add_argument("--single-vector", action="store_true",
+                    help="time FP64 one-vector applications, factored and explicit-C forms")
     ap.add_argument("--tiles", default="auto",
                     help="K1,K3 tile configuration indices (e.g. 0,0), or 'auto' to run the autotuner first")
     a = ap.parse_args(argv)
@@ -49,6 +57,8 @@ def main(argv=None):
     e1 = np.concatenate([np.sort(rng.uniform(-1.8, -0.5, a.nv)), np.sort(rng.uniform(-0.3, 4.0, a.nc))])
     ctx = IsdfContext(0)
     ctx.set_isdf(np.asfortranarray(psi), np.asfortranarray(mm), e1, a.nv)
+    if a.single_vector:
+        return single_vector_probe(a, ctx, psi, mm, e1)
     ctx.set_precision(a.precision)
     if a.precision != "f64":
         if a.tiles == "auto":
@@ -98,6 +108,42 @@ def calibration_copy(a, dev):
     dst.copy_(src)
     torch.cuda.synchronize()
     print(f"[probe] device copy of {n * 8} bytes (read {n * 8}, write {n * 8})", flush=True)
+
+
+def single_vector_probe(a, ctx, psi, mm, e1):
+    """FP64 one-vector applications: factored operator, then explicit C
+    (C(mu, icv) = Psi_c(mu) Psi_v(mu), icv = iv + nv ic)."""
+    import numpy as np
+    import torch
+
+    ncv = a.nv * a.nc
+    dev = torch.device("cuda:0")
+    x = torch.randn((ncv,), dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    s = np.sqrt((e1[a.nv:][:, None] - e1[:a.nv][None, :]).reshape(-1))
+    rows = []
+    for form in ("factored", "explicit"):
+        if form == "factored":
+            ctx.set_precision("f64")
+            nbytes = 8.0 * (a.nmu * (a.nv + a.nc) + a.nmu * a.nmu + 3 * ncv)
+        else:
+            cm = np.asfortranarray((psi[:, a.nv:][:, :, None] * psi[:, :a.nv][:, None, :]).reshape(a.nmu, ncv))
+            ctx.set_explicit_c(cm, mm, s)
+            del cm
+            nbytes = 8.0 * (2 * a.nmu * ncv + a.nmu * a.nmu + 3 * ncv)
+        torch.cuda.synchronize()
+        ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, 1, False)
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.reps):
+            ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, 1, False)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t) / a.reps
+        rows.append((form, dt, nbytes))
+        print(f"[probe] single vector FP64 {form} Nmu={a.nmu} nv={a.nv} nc={a.nc}: {a.reps} applications, "
+              f"{dt * 1e6:.1f} us each; {nbytes / 1e6:.1f} MB to read, {nbytes / dt / 1e12:.2f} TB/s", flush=True)
+    ctx.close()
+    return 0
 
 
 def lanczos_probe(a, ctx, e1, dev):
