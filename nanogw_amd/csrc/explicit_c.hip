@@ -4,6 +4,7 @@
 // tmpCmtrx(n_intp, ncv) with C(mu, icv) = Psi_v(mu) Psi_c(mu), and
 //   y = R^3/2 (R^1/2 x) + 4 R^1/2 C^T M C (R^1/2 x)       (full, :126-131)
 //   y = R x + 2 C^T M C x                                 (TDA)
+// Also the factored operator on one vector (operator_f64_one, below).
 // The three products are library GEMMs (rocBLAS dgemm) for several vectors;
 // for one vector (the reference's three dgemv calls) C x and M t run on the
 // column-split GEMV kernels below and C^T u on rocBLAS dgemv.  The diagonal
@@ -163,6 +164,96 @@ int operator_explicit(ngw_context* ctx, const OperatorArgs& a)
                                  nmu, u, nmu, &zero, y, (int)ncv), "dgemm C^T u"));
     }
     hipLaunchKernelGGL(explicit_epilogue_kernel, dim3(2048), dim3(256), 0, s, a.epilogue(ctx), y, ncv, B);
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
+}
+
+namespace {
+
+// t(mu) = sum_v Psi_v(mu, v) T1(mu, v)
+__global__ void rowdot_kernel(const double* psi, long ldp, const double* t1, int nmu, int nv, double* t)
+{
+    const int mu = blockIdx.x * 256 + threadIdx.x;
+    if (mu >= nmu) return;
+    double s = 0.0;
+    for (int v = 0; v < nv; ++v) s += psi[mu + (long)v * ldp] * t1[mu + (long)v * nmu];
+    t[mu] = s;
+}
+
+// W(mu, v) = Psi_v(mu, v) u(mu)
+__global__ void weight_kernel(const double* psi, long ldp, const double* u, int nmu, int nv, double* w)
+{
+    const long n = (long)nmu * nv;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int v = (int)(e / nmu), mu = (int)(e - (long)v * nmu);
+        w[e] = psi[mu + (long)v * ldp] * u[mu];
+    }
+}
+
+rocblas_handle blas(ngw_context* ctx, int& st)
+{
+    st = NGW_OK;
+    if (!ctx->rb_handle) {
+        rocblas_handle h = nullptr;
+        st = rb(rocblas_create_handle(&h), "rocblas_create_handle");
+        if (st != NGW_OK) return nullptr;
+        ctx->rb_handle = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(ctx->rb_handle);
+    st = rb(rocblas_set_stream(h, ctx->stream), "rocblas_set_stream");
+    return h;
+}
+
+}  // namespace
+
+// The factored operator on one vector.  The MFMA tiles of operator_f64 would
+// run nearly empty at N = nv columns, so the products are regrouped into
+// library GEMMs with the pair index split as (v, c), icv = v + nv c:
+//   T1 = Psi_c X^T          (n_intp x nv;  X(v, c) = (s o x)(icv))
+//   t(mu) = sum_v Psi_v(mu, v) T1(mu, v)             (= C (s o x))
+//   u = M t                                           (column-split GEMV)
+//   Y^T = (Psi_v o u)^T Psi_c   (nv x nc, Y^T(v, c) = (C^T u)(icv))
+// then the shared epilogue.
+int operator_f64_one(ngw_context* ctx, const OperatorArgs& a)
+{
+    hipStream_t s = ctx->stream;
+    const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond;
+    const long ncv = ctx->ncv, ldp = ctx->ld_psi;
+    int st = NGW_OK;
+    rocblas_handle h = blas(ctx, st);
+    NGW_TRY(st);
+    NGW_TRY(ctx->ex_x.ensure(sizeof(double) * (size_t)ncv));
+    NGW_TRY(ctx->ex_y.ensure(sizeof(double) * (size_t)ncv));
+    NGW_TRY(ctx->ex_t.ensure(sizeof(double) * (2 * (size_t)nmu * nv + 2 * (size_t)nmu)));
+    double* xs = ctx->ex_x.as<double>();
+    double* yt = ctx->ex_y.as<double>();
+    double* t1 = ctx->ex_t.as<double>();  // n_intp x nv
+    double* t = t1 + (size_t)nmu * nv;
+    double* u = t + nmu;
+    double* w = u + nmu;
+    const double* psi = ctx->psi.as<double>();
+    const double* psic = psi + (size_t)nv * ldp;
+    const double one = 1.0, zero = 0.0;
+    hipLaunchKernelGGL(scale_in_kernel, dim3(1024), dim3(256), 0, s, a.x, a.ldx,
+                       a.tda ? nullptr : ctx->sqrtr.as<double>(), ncv, 1, xs);
+    {
+        ProfScope ps(ctx, KC_K1);
+        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nmu, nv, nc, &one, psic,
+                                 (int)ldp, xs, nv, &zero, t1, nmu), "dgemm Psi_c X^T"));
+        hipLaunchKernelGGL(rowdot_kernel, dim3((nmu + 255) / 256), dim3(256), 0, s, psi, ldp, t1, nmu, nv, t);
+    }
+    {
+        ProfScope ps(ctx, KC_K2);
+        NGW_TRY(gemv_wide(s, ctx->mmat.as<double>(), ctx->ld_m, nmu, nmu, t, u, ctx->gv_part));
+    }
+    {
+        ProfScope ps(ctx, KC_K3);
+        hipLaunchKernelGGL(weight_kernel, dim3(std::min((nmu * (long)nv + 255) / 256, 2048L)), dim3(256), 0, s, psi,
+                           ldp, u, nmu, nv, w);
+        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, nc, nmu, &one, w, nmu,
+                                 psic, (int)ldp, &zero, yt, nv), "dgemm W^T Psi_c"));
+        hipLaunchKernelGGL(explicit_epilogue_kernel, dim3(1024), dim3(256), 0, s, a.epilogue(ctx), yt, ncv, 1);
+    }
     NGW_HIP(hipGetLastError());
     return NGW_OK;
 }

@@ -205,6 +205,7 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a)
 {
     const int nmu = ctx->n_intp, nv = ctx->nval, B = a.nvec;
     if (B <= 0) return NGW_OK;
+    if (B == 1) return operator_f64_one(ctx, a);
     const int S = k1_slots(nv);
     NGW_TRY(ctx->tpart.ensure(sizeof(double) * ((size_t)S + 1) * B * nmu));
     NGW_TRY(ctx->ubuf.ensure(sizeof(double) * (size_t)B * nmu));
@@ -215,10 +216,7 @@ int operator_f64(ngw_context* ctx, const OperatorArgs& a)
     NGW_TRY(launch_k1(ctx, a.x, a.ldx, sx, B, tpart, t));
     {
         ProfScope ps(ctx, KC_K2);
-        if (B == 1)  // one vector: u = M t is a GEMV
-            NGW_TRY(gemv_wide(ctx->stream, ctx->mmat.as<double>(), ctx->ld_m, nmu, nmu, t, u, ctx->gv_part));
-        else
-            NGW_TRY(launch_k2(ctx, ColMajorB{t, (long)nmu, B}, B, u));
+        NGW_TRY(launch_k2(ctx, ColMajorB{t, (long)nmu, B}, B, u));
     }
     {
         ProfScope ps(ctx, KC_K3);

@@ -117,6 +117,8 @@ std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const do
 int poly_apply(ngw_context* ctx, const double* q, double* out, double* h0, double* h1, double* h2, long ld,
                int nvec, const double* poly, int npoly, int tda, bool accumulate);
 int operator_f64(ngw_context* ctx, const OperatorArgs& a);
+// operator_f64 on one vector: library GEMMs on the (v, c) split of the pairs
+int operator_f64_one(ngw_context* ctx, const OperatorArgs& a);
 int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int nvec, int tda,
                  double* d_p, long ldp_out);
 
