@@ -2,11 +2,12 @@
 !
 ! Link this file into NanoGW instead of shared/apply_matvec_BLOCK.F90,
 ! shared/lanczos_spectra_BLOCK.F90, shared/apply_matvec.f90,
-! shared/lanczos_spectra.f90, shared/apply_matvec_UltraLowMem.F90 and
-! shared/lanczos_spectra_UltraLowMem.F90: the external subroutines keep the
+! shared/lanczos_spectra.f90, shared/apply_matvec_UltraLowMem.F90,
+! shared/lanczos_spectra_UltraLowMem.F90, shared/apply_matvec_lowcomsym.F90
+! and shared/lanczos_spectra_lowcomsym.F90: the external subroutines keep the
 ! reference's names and argument lists (implicit interfaces, arguments by
-! reference), so Zcalculate_sigma_lanczos_BLOCK, Zcalculate_sigma_lanczos and
-! Zcalculate_sigma_lanczos_UltraLowMem call them unchanged.
+! reference), so the Zcalculate_sigma_lanczos drivers (BLOCK, plain,
+! UltraLowMem, lowcomsym) call them unchanged.
 !
 ! BLOCK routines: the operator (Psi at the ISDF points, Mmtrx, energies) is
 ! uploaded to the GPU once and kept resident; it is re-uploaded when the
@@ -19,6 +20,10 @@
 ! UltraLowMem routines (matvec_isdf_UltraLowMem and its polynomial and
 ! Lanczos companions): single vectors on the BLOCK routines' resident
 ! operator, same scope.
+!
+! lowcomsym routines (CPU-build argument lists): the irrep-blocked operator
+! uploaded as an explicit C over the pairs of representation nmrep, built
+! here in the reference's pair order (see ngw_lowcomsym_operator).
 !
 ! Single-vector routines (matvec_isdf, polynomial_matvec_isdf,
 ! lanczos_spectra_isdf): the caller's tmpCmtrx, Mmtrx and sqrtR are uploaded
@@ -53,6 +58,11 @@ module ngw_session
   type(c_ptr), save :: up_c = c_null_ptr, up_mx = c_null_ptr
   integer, save :: up_cnintp = -1, up_cncv = -1
   real(c_double), allocatable, save :: up_sqrtr(:), up_csamp(:), up_msamp(:)
+  ! irrep-blocked (lowcomsym) explicit operator: keyed by its Psi blocks,
+  ! Mmtrx_loc, nmrep and samples
+  type(c_ptr), save :: up_lv = c_null_ptr, up_lcp = c_null_ptr, up_lm = c_null_ptr
+  integer, save :: up_lnmrep = -1, up_lncv = -1, up_lnintp = -1
+  real(c_double), allocatable, save :: up_lsqrtr(:), up_lsamp(:)
 
 contains
 
@@ -104,6 +114,7 @@ contains
     up_psi = c_null_ptr
     up_m = c_null_ptr
     up_nintp = -1
+    up_lncv = -1
   end subroutine ngw_ensure_explicit
 
   ! nval and nband from the pair list; checks NanoGW's all-pairs ordering.
@@ -146,6 +157,7 @@ contains
     up_nval = nval
     up_c = c_null_ptr
     up_cnintp = -1
+    up_lncv = -1
   end subroutine ngw_ensure
 
   ! The single-vector reference Lanczos routines leave the recurrence at the
@@ -199,6 +211,45 @@ contains
     deallocate (a_elmt, b_elmt)
   end subroutine ngw_lanczos_single
 
+  ! The irrep-blocked operator of matvec_isdf_lowcomsym, uploaded in explicit
+  ! form unless the same one is resident.  cm(n_intp, ncv) holds the pair
+  ! products in the reference's order; key_* identify the caller's arrays
+  ! (Psi_v and Psi_c blocks, Mmtrx_loc) and samp a sample of their values.
+  logical function ngw_lowcomsym_resident(kv, kc, km, nmrep, n_intp, ncv, sqrtr, samp)
+    type(c_ptr), intent(in) :: kv, kc, km
+    integer, intent(in) :: nmrep, n_intp, ncv
+    real(c_double), intent(in) :: sqrtr(ncv), samp(:)
+    ngw_lowcomsym_resident = .false.
+    if (.not. c_associated(ngw_ctx)) return
+    if (up_lncv /= ncv .or. up_lnintp /= n_intp .or. up_lnmrep /= nmrep) return
+    if (.not. (c_associated(up_lv, kv) .and. c_associated(up_lcp, kc) .and. c_associated(up_lm, km))) return
+    if (size(up_lsamp) /= size(samp)) return
+    ngw_lowcomsym_resident = all(up_lsqrtr == sqrtr) .and. all(up_lsamp == samp)
+  end function ngw_lowcomsym_resident
+
+  subroutine ngw_upload_lowcomsym(kv, kc, km, nmrep, n_intp, ncv, cm, ld_m, mmtrx, sqrtr, samp)
+    type(c_ptr), intent(in) :: kv, kc, km
+    integer, intent(in) :: nmrep, n_intp, ncv, ld_m
+    real(c_double), intent(in) :: cm(n_intp, ncv), mmtrx(ld_m, n_intp), sqrtr(ncv), samp(:)
+    call ngw_start()
+    call ngw_check(ngw_set_explicit_c(ngw_ctx, int(n_intp, c_int), int(ncv, c_int), cm, int(n_intp, c_int), &
+                                      mmtrx, int(ld_m, c_int), sqrtr), 'ngw_set_explicit_c (lowcomsym)')
+    up_lv = kv
+    up_lcp = kc
+    up_lm = km
+    up_lnmrep = nmrep
+    up_lncv = ncv
+    up_lnintp = n_intp
+    up_lsqrtr = sqrtr
+    up_lsamp = samp
+    ! the other operator forms are gone from the context
+    up_psi = c_null_ptr
+    up_m = c_null_ptr
+    up_nintp = -1
+    up_c = c_null_ptr
+    up_cnintp = -1
+  end subroutine ngw_upload_lowcomsym
+
   subroutine ngw_release()
     integer(c_int) :: st
     if (c_associated(ngw_ctx)) st = ngw_destroy(ngw_ctx)
@@ -209,6 +260,7 @@ contains
     up_c = c_null_ptr
     up_mx = c_null_ptr
     up_cnintp = -1
+    up_lncv = -1
   end subroutine ngw_release
 
 end module ngw_session
@@ -414,3 +466,126 @@ subroutine lanczos_spectra_isdf_UltraLowMem(v0, v0_norm, ncv_loc, zz, nz, niter,
   call ngw_lanczos_single(v0, v0_norm, ncv_loc, zz, nz, niter, polynomial, npoly, spectra, &
                           'lanczos_spectra_isdf_UltraLowMem')
 end subroutine lanczos_spectra_isdf_UltraLowMem
+
+! Explicit pair matrix of the irrep-blocked operator in the pair order of
+! matvec_isdf_lowcomsym (shared/apply_matvec_lowcomsym.F90:155-247): for each
+! valence irrep jvrep the conduction irrep jcrep with prod(jcrep, jvrep) =
+! nmrep; valence states of jvrep outer, conduction states of jcrep inner;
+! column = PsiC_intp_bl(:, lcrep(ic)) * PsiV_intp_bl(:, lvrep(iv)).  The
+! operator (a single rank, w_grp%npes = 1) is uploaded unless resident.
+subroutine ngw_lowcomsym_operator(ncv_loc, isdf_in, sqrtR, nmrep, gvec)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, nmrep
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(in) :: sqrtR(ncv_loc)
+  type(gspace), intent(in) :: gvec
+  real(dp), allocatable :: cm(:, :), samp(:)
+  integer :: n, jvrep, jcrep, iv, ic, icv, k, nv_bl, nc_bl
+  type(c_ptr) :: kv, kc, km
+  if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
+  n = isdf_in%n_intp_r
+  nv_bl = size(isdf_in%PsiV_intp_bl, 2)
+  nc_bl = size(isdf_in%PsiC_intp_bl, 2)
+  kv = c_loc(isdf_in%PsiV_intp_bl(1, 1, 1, 1))
+  kc = c_loc(isdf_in%PsiC_intp_bl(1, 1, 1, 1))
+  km = c_loc(isdf_in%Mmtrx_loc(1, 1, 1, 1, 1, 1, nmrep))
+  allocate (samp(0))
+  do k = 1, min(n * nv_bl, 1024)
+    samp = [samp, isdf_in%PsiV_intp_bl(1 + mod((k - 1) * 7919, n), 1 + mod((k - 1) * 104729, nv_bl), 1, 1)]
+  end do
+  do k = 1, min(n * nc_bl, 1024)
+    samp = [samp, isdf_in%PsiC_intp_bl(1 + mod((k - 1) * 7919, n), 1 + mod((k - 1) * 104729, nc_bl), 1, 1)]
+  end do
+  do k = 1, min(n, 1024)
+    samp = [samp, isdf_in%Mmtrx_loc(k, 1 + mod((k - 1) * 7919, n), 1, 1, 1, 1, nmrep)]
+  end do
+  if (ngw_lowcomsym_resident(kv, kc, km, nmrep, n, ncv_loc, sqrtR, samp)) return
+  allocate (cm(n, ncv_loc))
+  icv = 0
+  do jvrep = 1, gvec%syms%ntrans
+    do jcrep = 1, gvec%syms%ntrans
+      if (gvec%syms%prod(jcrep, jvrep) == nmrep) exit
+    end do
+    if (jcrep > gvec%syms%ntrans) cycle
+    do iv = isdf_in%lvrep_bound(jvrep, 1), isdf_in%lvrep_bound(jvrep, 2)
+      do ic = isdf_in%lcrep_bound(jcrep, 1), isdf_in%lcrep_bound(jcrep, 2)
+        icv = icv + 1
+        if (icv > ncv_loc) stop 'libngw drop-in: lowcomsym pair count exceeds ncv_loc'
+        cm(:, icv) = isdf_in%PsiC_intp_bl(1:n, isdf_in%lcrep(ic), 1, 1) * &
+                     isdf_in%PsiV_intp_bl(1:n, isdf_in%lvrep(iv), 1, 1)
+      end do
+    end do
+  end do
+  if (icv /= ncv_loc) stop 'libngw drop-in: lowcomsym pair count differs from ncv_loc'
+  call ngw_upload_lowcomsym(kv, kc, km, nmrep, n, ncv_loc, cm, size(isdf_in%Mmtrx_loc, 1), &
+                            isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, nmrep), sqrtR, samp)
+  deallocate (cm, samp)
+end subroutine ngw_lowcomsym_operator
+
+! shared/apply_matvec_lowcomsym.F90:31 (CPU build's argument list)
+subroutine matvec_isdf_lowcomsym(vec, Hvec, ncv_loc, isdf_in, sqrtR, tamm_d, blksz, nmrep, gvec)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, blksz, nmrep
+  type(gspace), intent(in) :: gvec
+  real(dp), intent(in) :: vec(ncv_loc, blksz), sqrtR(ncv_loc)
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(out) :: Hvec(ncv_loc, blksz)
+  logical, intent(in) :: tamm_d
+  integer(c_int) :: td
+  call ngw_lowcomsym_operator(ncv_loc, isdf_in, sqrtR, nmrep, gvec)
+  td = 0
+  if (tamm_d) td = 1
+  call ngw_check(ngw_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), int(blksz, c_int), td), &
+                 'matvec_isdf_lowcomsym')
+end subroutine matvec_isdf_lowcomsym
+
+! shared/apply_matvec_lowcomsym.F90:419 (always the full operator, :478-483)
+subroutine polynomial_matvec_isdf_lowcomsym(vec, Hvec, ncv_loc, polynomial, npoly, isdf_in, sqrtR, &
+                                            blksz, nmrep, gvec)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, npoly, blksz, nmrep
+  type(gspace), intent(in) :: gvec
+  real(dp), intent(in) :: vec(ncv_loc, blksz), polynomial(npoly + 1), sqrtR(ncv_loc)
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(out) :: Hvec(ncv_loc, blksz)
+  call ngw_lowcomsym_operator(ncv_loc, isdf_in, sqrtR, nmrep, gvec)
+  call ngw_check(ngw_polynomial_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), polynomial, &
+                                                  int(npoly, c_int), int(blksz, c_int), 0_c_int), &
+                 'polynomial_matvec_isdf_lowcomsym')
+end subroutine polynomial_matvec_isdf_lowcomsym
+
+! shared/lanczos_spectra_lowcomsym.F90:7.  TDA runs the recurrence on the TDA
+! operator itself (:76-86, :130-140), the full mode on p(H) (:87-97); no
+! breakdown exit (:120).
+subroutine lanczos_spectra_isdf_lowcomsym(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_in, sqrtR, polynomial, npoly, &
+                                          spectra, tamm_d, blksz, nmrep, gvec)
+  NGW_TYPES
+  use ngw_session
+  implicit none
+  integer, intent(in) :: ncv_loc, niter, nz, npoly, blksz, nmrep
+  complex(dp), intent(in) :: zz(nz, blksz)
+  type(ISDF), intent(in) :: isdf_in
+  real(dp), intent(in) :: v0(ncv_loc, blksz), polynomial(npoly + 1), v0_norm(blksz), sqrtR(ncv_loc)
+  type(gspace), intent(in) :: gvec
+  logical, intent(in) :: tamm_d
+  complex(dp), intent(out) :: spectra(nz, blksz)
+  real(c_double) :: identity(2)
+  call ngw_lowcomsym_operator(ncv_loc, isdf_in, sqrtR, nmrep, gvec)
+  if (tamm_d) then
+    identity = [1.d0, 0.d0]
+    call ngw_check(ngw_lanczos_spectra_isdf_block(ngw_ctx, v0, v0_norm, int(ncv_loc, c_int), zz, int(nz, c_int), &
+                                                  int(niter, c_int), identity, 1_c_int, spectra, int(blksz, c_int), &
+                                                  1_c_int, c_null_ptr, c_null_ptr), 'lanczos_spectra_isdf_lowcomsym')
+  else
+    call ngw_check(ngw_lanczos_spectra_isdf_block(ngw_ctx, v0, v0_norm, int(ncv_loc, c_int), zz, int(nz, c_int), &
+                                                  int(niter, c_int), polynomial, int(npoly, c_int), spectra, &
+                                                  int(blksz, c_int), 0_c_int, c_null_ptr, c_null_ptr), &
+                 'lanczos_spectra_isdf_lowcomsym')
+  end if
+end subroutine lanczos_spectra_isdf_lowcomsym

@@ -4,8 +4,9 @@
 ! lanczos_spectra_isdf_BLOCK as Zcalculate_sigma_lanczos_BLOCK does, then the
 ! single-vector matvec_isdf, polynomial_matvec_isdf and lanczos_spectra_isdf
 ! as Zcalculate_sigma_lanczos does (tmp_Cmtrx built from Psi, here over the
-! pair list in reverse order), then their UltraLowMem variants, and writes
-! the results to a stream file (argument 2).
+! pair list in reverse order), then their UltraLowMem variants, then the
+! lowcomsym routines on a two-irrep blocking, and writes the results to a
+! stream file (argument 2).
 program ngw_dropin_check
   use ngw_dropin_selftest_types
   use ngw_session, only: ngw_release
@@ -19,6 +20,10 @@ program ngw_dropin_check
   complex(dp), allocatable :: zz(:, :), spectra(:, :), spec1(:), speig(:)
   integer :: iv, ic, jcv
   integer, allocatable :: cvpair2(:, :)
+  type(gspace) :: gvec
+  integer :: nv1, nc1, nsub, jr, k
+  real(dp), allocatable :: sqrtrs(:), vs(:, :), hs(:, :), hst(:, :), ps(:, :), vsn(:)
+  complex(dp), allocatable :: specs(:, :), spect(:, :), zzs(:, :)
   real(dp), allocatable :: hu(:), pu(:), veig2(:)
   complex(dp), allocatable :: specu(:), speigu(:)
   character(len=512) :: fin, fout
@@ -79,8 +84,50 @@ program ngw_dropin_check
                                         sqrtR, poly, npoly, specu, dummy, dummy, dummy, dummy, tmp_cvpair)
   call lanczos_spectra_isdf_UltraLowMem(veig2, sqrt(sum(veig2**2)), ncv, zz(:, 1), nz, niter, isdf_in, wfn, &
                                         cvpair2, sqrtR, poly, npoly, speigu, dummy, dummy, dummy, dummy, tmp_cvpair)
+  ! lowcomsym: irreps 1 and 2 with the Z2 product table, nmrep = 1 (pairs
+  ! within one irrep); valence 1..nv/2 in irrep 1, the rest in irrep 2;
+  ! conduction states alternate, odd in irrep 1
+  nv1 = nv / 2
+  nc1 = (nc + 1) / 2
+  allocate (isdf_in%PsiV_intp_bl(nmu, nv, 1, 1), isdf_in%PsiC_intp_bl(nmu, nc, 1, 1))
+  isdf_in%PsiV_intp_bl(:, :, 1, 1) = isdf_in%Psi_intp_loc(:, 1:nv, 1, 1)
+  isdf_in%PsiC_intp_bl(:, :, 1, 1) = isdf_in%Psi_intp_loc(:, nv + 1:nb, 1, 1)
+  allocate (isdf_in%lvrep(nv), isdf_in%lcrep(nc), isdf_in%lvrep_bound(2, 2), isdf_in%lcrep_bound(2, 2))
+  isdf_in%lvrep = [(iv, iv = 1, nv)]
+  isdf_in%lvrep_bound(1, :) = [1, nv1]
+  isdf_in%lvrep_bound(2, :) = [nv1 + 1, nv]
+  isdf_in%lcrep = [(2 * ic - 1, ic = 1, nc1), (2 * ic, ic = 1, nc / 2)]
+  isdf_in%lcrep_bound(1, :) = [1, nc1]
+  isdf_in%lcrep_bound(2, :) = [nc1 + 1, nc]
+  allocate (gvec%syms%prod(2, 2))
+  gvec%syms%ntrans = 2
+  gvec%syms%prod = reshape([1, 2, 2, 1], [2, 2])
+  nsub = nv1 * nc1 + (nv - nv1) * (nc / 2)
+  allocate (sqrtrs(nsub), vs(nsub, 2), hs(nsub, 2), hst(nsub, 2), ps(nsub, 2), vsn(2))
+  allocate (specs(nz, 2), spect(nz, 2), zzs(nz, 2))
+  icv = 0
+  do jr = 1, 2
+    do iv = isdf_in%lvrep_bound(jr, 1), isdf_in%lvrep_bound(jr, 2)
+      do ic = isdf_in%lcrep_bound(jr, 1), isdf_in%lcrep_bound(jr, 2)
+        icv = icv + 1
+        sqrtrs(icv) = sqrt(wfn%e1(nv + isdf_in%lcrep(ic)) - wfn%e1(isdf_in%lvrep(iv)))
+      end do
+    end do
+  end do
+  do k = 1, 2
+    vs(:, k) = vec(1:nsub, k)
+    zzs(:, k) = zz(:, k)
+  end do
+  vsn = sqrt(sum(vs**2, dim=1))
+  call matvec_isdf_lowcomsym(vs, hs, nsub, isdf_in, sqrtrs, .false., 2, 1, gvec)
+  call matvec_isdf_lowcomsym(vs, hst, nsub, isdf_in, sqrtrs, .true., 2, 1, gvec)
+  call polynomial_matvec_isdf_lowcomsym(vs, ps, nsub, poly, npoly, isdf_in, sqrtrs, 2, 1, gvec)
+  call lanczos_spectra_isdf_lowcomsym(vs, vsn, nsub, zzs, nz, niter, isdf_in, sqrtrs, poly, npoly, specs, .false., &
+                                      2, 1, gvec)
+  call lanczos_spectra_isdf_lowcomsym(vs, vsn, nsub, zzs, nz, niter, isdf_in, sqrtrs, poly, npoly, spect, .true., &
+                                      2, 1, gvec)
   call ngw_release()
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u) hvec, spectra, h1, p1, spec1, speig, hu, pu, specu, speigu
+  write (u) hvec, spectra, h1, p1, spec1, speig, hu, pu, specu, speigu, hs, hst, ps, specs, spect
   close (u)
 end program ngw_dropin_check

@@ -1,8 +1,9 @@
 """The Fortran drop-in (nanogw_amd/fortran/ngw_dropin.F90, NanoGW's
 matvec_isdf_BLOCK / lanczos_spectra_isdf_BLOCK, single-vector
-matvec_isdf / polynomial_matvec_isdf / lanczos_spectra_isdf and their
-UltraLowMem variants, with the reference names and argument lists over
-iso_c_binding) called from a Fortran program, against the CPU oracle."""
+matvec_isdf / polynomial_matvec_isdf / lanczos_spectra_isdf, their
+UltraLowMem variants and the lowcomsym routines, with the reference names and
+argument lists over iso_c_binding) called from a Fortran program, against the
+CPU oracle."""
 import os
 import subprocess
 
@@ -56,8 +57,22 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     speig = raw[o2 + 2 * ncv + 2 * nz:o3].view(np.complex128)
     hu, pu = raw[o3:o3 + ncv], raw[o3 + ncv:o3 + 2 * ncv]
     specu = raw[o3 + 2 * ncv:o3 + 2 * ncv + 2 * nz].view(np.complex128)
-    speigu = raw[o3 + 2 * ncv + 2 * nz:].view(np.complex128)
-    assert len(speigu) == nz
+    o4 = o3 + 2 * ncv + 4 * nz
+    speigu = raw[o3 + 2 * ncv + 2 * nz:o4].view(np.complex128)
+    # lowcomsym blocking of the check program: irrep 1 = valence 0..nv/2-1 and
+    # even conduction indices, irrep 2 = the rest; pairs within one irrep,
+    # valence outer, conduction inner
+    nv1, nc1 = nv // 2, (nc + 1) // 2
+    cl = list(range(0, nc, 2)) + list(range(1, nc, 2))
+    pairs = [(v, cl[c]) for v in range(nv1) for c in range(nc1)]
+    pairs += [(v, cl[c]) for v in range(nv1, nv) for c in range(nc1, nc)]
+    nsub = len(pairs)
+    blk = raw[o4:].reshape(-1)
+    hs = blk[:2 * nsub].reshape(2, nsub).T
+    hst = blk[2 * nsub:4 * nsub].reshape(2, nsub).T
+    ps = blk[4 * nsub:6 * nsub].reshape(2, nsub).T
+    specs = blk[6 * nsub:6 * nsub + 4 * nz].view(np.complex128).reshape(2, nz).T
+    spect = blk[6 * nsub + 4 * nz:].view(np.complex128).reshape(2, nz).T
     ref_h = O.matvec_block(m.psi_intp, m.mmtrx, s, vec, nv, nc)
     ref_s, _, _ = O.lanczos_spectra_block(m.psi_intp, m.mmtrx, s, vec, np.linalg.norm(vec, axis=0), zz, niter,
                                           poly, nv, nc)
@@ -83,3 +98,26 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
                                          nv, nc)
     assert rel_err(specu, refu[:, 0]) < 1e-6
     assert rel_err(speigu, 1.0 / (zz[:, 0] - lam)) < 1e-9
+    # lowcomsym: the operator on a pair subset is the principal submatrix of
+    # the all-pairs operator; Horner and Lanczos against the single-vector
+    # restatements on the subset's Cmtrx; the TDA Lanczos runs on H itself
+    idx = np.array([v + nv * c for v, c in pairs])
+    xs = vec[:nsub, :2]
+
+    def sub_apply(x, tda):
+        xf = np.zeros((ncv, x.shape[1]))
+        xf[idx] = x
+        return O.matvec_block(m.psi_intp, m.mmtrx, s, xf, nv, nc, tda)[idx]
+
+    assert rel_err(hs, sub_apply(xs, False)) < 1e-11
+    assert rel_err(hst, sub_apply(xs, True)) < 1e-11
+    cms = np.asfortranarray(O.pair_cmtrx(m.psi_intp, nv, nc)[:, idx])
+    for k in range(2):
+        assert rel_err(ps[:, k], O.polynomial_matvec_explicit(cms, m.mmtrx, s[idx], xs[:, k], poly)) < 1e-10
+        ref, _, _, nk = O.lanczos_spectra_explicit(cms, m.mmtrx, s[idx], xs[:, k], np.linalg.norm(xs[:, k]),
+                                                   zz[:, k], niter, poly)
+        assert nk == niter and rel_err(specs[:, k], ref) < 1e-6
+        htda = sub_apply(np.eye(nsub), True)
+        reft, _, _ = O.lanczos_dense(0.5 * (htda + htda.T), xs[:, k], zz[:, k], niter)
+        print("lowcomsym TDA lanczos rel err", rel_err(spect[:, k], reft))
+        assert rel_err(spect[:, k], reft) < 1e-6
