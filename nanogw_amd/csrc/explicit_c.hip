@@ -117,6 +117,21 @@ int rb(rocblas_status st, const char* what)
     return NGW_OK;
 }
 
+// The context's rocBLAS handle (created on first use) bound to its stream.
+rocblas_handle blas(ngw_context* ctx, int& st)
+{
+    st = NGW_OK;
+    if (!ctx->rb_handle) {
+        rocblas_handle h = nullptr;
+        st = rb(rocblas_create_handle(&h), "rocblas_create_handle");
+        if (st != NGW_OK) return nullptr;
+        ctx->rb_handle = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(ctx->rb_handle);
+    st = rb(rocblas_set_stream(h, ctx->stream), "rocblas_set_stream");
+    return h;
+}
+
 }  // namespace
 
 int operator_explicit(ngw_context* ctx, const OperatorArgs& a)
@@ -127,13 +142,9 @@ int operator_explicit(ngw_context* ctx, const OperatorArgs& a)
     hipStream_t s = ctx->stream;
     const long ncv = ctx->ncv;
     const int nmu = ctx->n_intp;
-    if (!ctx->rb_handle) {
-        rocblas_handle h = nullptr;
-        NGW_TRY(rb(rocblas_create_handle(&h), "rocblas_create_handle"));
-        ctx->rb_handle = h;
-    }
-    rocblas_handle h = static_cast<rocblas_handle>(ctx->rb_handle);
-    NGW_TRY(rb(rocblas_set_stream(h, s), "rocblas_set_stream"));
+    int st = NGW_OK;
+    rocblas_handle h = blas(ctx, st);
+    NGW_TRY(st);
     NGW_TRY(ctx->ex_x.ensure(sizeof(double) * (size_t)ncv * B));
     NGW_TRY(ctx->ex_y.ensure(sizeof(double) * (size_t)ncv * B));
     NGW_TRY(ctx->ex_t.ensure(sizeof(double) * 2 * (size_t)nmu * B));
@@ -188,20 +199,6 @@ __global__ void weight_kernel(const double* psi, long ldp, const double* u, int 
         const int v = (int)(e / nmu), mu = (int)(e - (long)v * nmu);
         w[e] = psi[mu + (long)v * ldp] * u[mu];
     }
-}
-
-rocblas_handle blas(ngw_context* ctx, int& st)
-{
-    st = NGW_OK;
-    if (!ctx->rb_handle) {
-        rocblas_handle h = nullptr;
-        st = rb(rocblas_create_handle(&h), "rocblas_create_handle");
-        if (st != NGW_OK) return nullptr;
-        ctx->rb_handle = h;
-    }
-    rocblas_handle h = static_cast<rocblas_handle>(ctx->rb_handle);
-    st = rb(rocblas_set_stream(h, ctx->stream), "rocblas_set_stream");
-    return h;
 }
 
 }  // namespace
