@@ -239,7 +239,7 @@ int ngw_destroy(ngw_context* ctx)
                       &ctx->tpart, &ctx->ubuf, &ctx->w0, &ctx->w1, &ctx->w2, &ctx->w3, &ctx->w4,
                       &ctx->ab, &ctx->zz, &ctx->spec, &ctx->misc, &ctx->misc2, &ctx->staging,
                       &ctx->pct32, &ctx->pc32, &ctx->pctb, &ctx->pcb, &ctx->pcth, &ctx->pch, &ctx->m32t,
-                      &ctx->psiv32, &ctx->psiv_scale, &ctx->xsig, &ctx->lpwork, &ctx->gv_part};
+                      &ctx->psiv32, &ctx->psivt, &ctx->psiv_scale, &ctx->xsig, &ctx->lpwork, &ctx->gv_part};
     for (DevBuf* b : bufs) b->release();
     explicit_release(ctx);
     for (auto& r : ctx->prof_recs) {

@@ -6,7 +6,8 @@
 //   per system (once):  pct = (Psi_c / sigma_c)^T  [mu][c]   K1 A operand
 //                       pc  =  Psi_c / sigma_c     [c][mu]   K3 A operand
 //                       m32t = M^T FP32            [mu][nu]  K2 A operand
-//                       psiv32 = Psi_v^T FP32      [v][mu]   K1 epilogue, K3 B factor
+//                       psiv32 = Psi_v^T FP32      [v][mu]   wprep (K3 B factor)
+//                       psivt  = Psi_v FP32        [mu][v]   K1 epilogue
 //   per operator application:
 //     xprep  x'(j, c) = sigma_c s(v,c) x_b(v,c)   (transpose through LDS)
 //     K1     T = pct x'^T, epilogue * Psi_v and reduce over v (LDS)
@@ -146,6 +147,17 @@ __global__ void prep_m_kernel(const double* m, long ldm, int nmu, int ldm1, cons
     }
 }
 
+// psivt(mu, v) = Psi_v(mu, v) in FP32, v contiguous: the K1 epilogue reads
+// 16 consecutive v of one row per wave quarter instead of 16 rows.
+__global__ void psivt_kernel(const double* psi, long ldp, int nmu, int nv, float* psivt)
+{
+    const long n = (long)nmu * nv;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int mu = (int)(e / nv), v = (int)(e - (long)mu * nv);
+        psivt[e] = (float)psi[mu + (long)v * ldp];
+    }
+}
+
 // Maximum of one value per thread over a 256-thread workgroup (all threads).
 __device__ __forceinline__ float block_max256(float m)
 {
@@ -202,6 +214,7 @@ int lp2_prepare(ngw_context* ctx)
     NGW_TRY(ctx->xsig.ensure(sizeof(float) * 2 * (size_t)ctx->ncv));
     NGW_TRY(ctx->m32t.ensure(4 * (size_t)(nmu + 1) * ctx->ldm1));
     NGW_TRY(ctx->psiv32.ensure(4 * (size_t)nv * ctx->ldm1));
+    NGW_TRY(ctx->psivt.ensure(4 * (size_t)nmu * nv));
     hipStream_t s = ctx->stream;
     const double* psi = ctx->psi.as<double>();
     const float* sig = ctx->psi_scale.as<float>();
@@ -215,6 +228,8 @@ int lp2_prepare(ngw_context* ctx)
                        ctx->ldc1, ctx->ldm1, (void*)ctx->pcth.ptr, (void*)nullptr, (void*)ctx->pch.ptr, (void*)nullptr);
     hipLaunchKernelGGL(prep_m_kernel, dim3(2048), dim3(256), 0, s, ctx->mmat.as<double>(), (long)ctx->ld_m, nmu,
                        ctx->ldm1, psi, (long)ctx->ld_psi, nv, ctx->m32t.as<float>(), ctx->psiv32.as<float>());
+    hipLaunchKernelGGL(psivt_kernel, dim3(1024), dim3(256), 0, s, psi, (long)ctx->ld_psi, nmu, nv,
+                       ctx->psivt.as<float>());
     hipLaunchKernelGGL(psiv_scale_kernel, dim3(nv), dim3(256), 0, s, ctx->psiv32.as<float>(), ctx->ldm1, nmu,
                        ctx->psiv_scale.as<float>());
     hipLaunchKernelGGL(xsig_kernel, dim3(1024), dim3(256), 0, s, ctx->sqrtr.as<double>(), sig, nv, ctx->ncv,
@@ -526,7 +541,7 @@ __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, co
             if (row < pass * HALF || row >= (pass + 1) * HALF) return;
             int gm = m0 + row, gn = n0 + col;
             float p = 0.0f;
-            if (gm < op.nmu && gn < N) p = val * op.psiv32[gm + (long)(gn % nv) * op.ldm1];
+            if (gm < op.nmu && gn < N) p = val * op.psivt[(long)gm * nv + gn % nv];
             sm.red[row - pass * HALF][col] = p;
         });
         __syncthreads();
@@ -861,7 +876,7 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
         d.pct_hi = op->pctb.ptr; d.pct_lo = op->pctb.as<unsigned short>() + n1;
         d.pc_hi = op->pcb.ptr; d.pc_lo = op->pcb.as<unsigned short>() + n2;
     }
-    d.m32t = op->m32t.as<float>(); d.psiv32 = op->psiv32.as<float>();
+    d.m32t = op->m32t.as<float>(); d.psiv32 = op->psiv32.as<float>(); d.psivt = op->psivt.as<float>();
     d.psi = op->psi.as<double>(); d.ldp = op->ld_psi;
     d.sig = op->psi_scale.as<float>();
     d.xsig = op->xsig.as<float>() + (tda ? op->ncv : 0);

@@ -87,7 +87,7 @@ struct ngw_context {
     // pre-converted operands of the streamlined low-precision path
     bool lp2_ready = false;
     int ldc1 = 0, ldm1 = 0;
-    ngw::DevBuf pct32, pc32, pctb, pcb, pcth, pch, m32t, psiv32, psiv_scale, xsig, lpwork;
+    ngw::DevBuf pct32, pc32, pctb, pcb, pcth, pch, m32t, psiv32, psivt, psiv_scale, xsig, lpwork;
     // chosen {K1, K3} tile configuration per precision mode (ngw_autotune)
     int lp_cfg[ngw::PREC_COUNT][2] = {};
     // explicit pair matrix C (n_intp x ncv, ngw_set_explicit_c) in place of

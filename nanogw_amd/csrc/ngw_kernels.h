@@ -138,6 +138,7 @@ struct LpOp {
     const void* pct_hi; const void* pct_lo;
     const void* pc_hi; const void* pc_lo;
     const float* m32t; const float* psiv32;
+    const float* psivt;  // Psi_v FP32 [mu][v] (K1 epilogue: one row's v contiguous)
     const double* psi; long ldp;
     const float* sig;
     const float* xsig;  // x' = xsig o x: sigma_c sqrtR(v,c), or sigma_c for TDA (ncv)
