@@ -52,6 +52,7 @@ def test_null_context_rejected():
     assert lib.ngw_synchronize(None) == -1
     assert lib.ngw_matvec_isdf_block(None, None, None, 1, 1, 0) == -1
     assert lib.ngw_set_precision(None, 0) == -1
+    assert lib.ngw_set_explicit_c(None, 4, 4, None, 4, None, 4, None) == -1
 
 
 def test_header_documents_reference_interfaces():
@@ -61,5 +62,5 @@ def test_header_documents_reference_interfaces():
     for ref in ["apply_matvec_BLOCK.F90:25", "apply_matvec_BLOCK.F90:210",
                 "lanczos_spectra_BLOCK.F90:7", "calculate_sigma_lanczos_BLOCK.F90z:201-444",
                 "printsigma.F90:97-128", "polyfit.f90:1", "isdf_parallel_sym_UltraLowMem.F90",
-                "fft_module.F90z:122-177"]:
+                "fft_module.F90z:122-177", "apply_matvec.f90:25", "lanczos_spectra.f90:7"]:
         assert ref in text, ref
