@@ -312,3 +312,40 @@ def test_explicit_lanczos_equals_block_and_breakdown(O, tiny):
     assert nk == 1 and abs(b[1]) < 1e-8 and np.all(a[1:] == 0.0)
     assert abs(a[0] - lam) < 1e-10 * abs(lam)
     assert rel_err(spec, 1.0 / (zz - lam)) < 1e-10
+
+
+def test_operator_dominates_r_squared(O, tiny):
+    """With M positive semi-definite, H - R^2 = 4 S C^T M C S is PSD, so
+    x^T H x >= x^T R^2 x for every x (a wrong sign or a dropped S breaks it)."""
+    H, s, _ = _dense_h(tiny)
+    w = np.linalg.eigvalsh(0.5 * (tiny.mmtrx + tiny.mmtrx.T))
+    assert w.min() > -1e-10 * abs(w).max()
+    x = np.random.default_rng(41).normal(size=(tiny.nv * tiny.nc, 8))
+    y = O.matvec_block(tiny.psi_intp, tiny.mmtrx, s, x, tiny.nv, tiny.nc)
+    quad = np.einsum("ij,ij->j", x, y)
+    base = np.einsum("ij,ij->j", x, (s ** 4)[:, None] * x)
+    assert np.all(quad >= base * (1 - 1e-12))
+    assert np.linalg.eigvalsh(H - np.diag(s ** 4)).min() > -1e-9 * np.abs(H).max()
+
+
+def test_isdf_fit_residual_decreases_with_points():
+    """The interpolative fit of the pair densities psi_v psi_c improves as
+    interpolation points are added (points from the same CVT growth), and is
+    exact at the points themselves."""
+    from nanogw_amd import synth
+
+    gidx = synth.sphere_grid(4.0, 0.5)
+    rng = np.random.default_rng(7)
+    nv, nc = 3, 6
+    psi = synth.orbitals(gidx, 0.5, nv + nc, rng, 4.0)
+    pairs = (psi[:, nv:][:, :, None] * psi[:, :nv][:, None, :]).reshape(len(gidx), -1)
+    rho = 2.0 * np.sum(psi[:, :nv] ** 2, axis=1)
+    res = []
+    for n_intp in (6, 12, 18):
+        intp = synth.cvt_points(gidx, 0.5, rho, n_intp, np.random.default_rng(11))
+        zeta = synth.isdf_zeta(psi, intp, nv)
+        fit = zeta @ pairs[intp, :]
+        assert np.abs(fit[intp] - pairs[intp]).max() < 1e-8 * np.abs(pairs).max()
+        res.append(np.linalg.norm(fit - pairs) / np.linalg.norm(pairs))
+    print("ISDF relative residuals", res)
+    assert res[0] > res[1] > res[2]
