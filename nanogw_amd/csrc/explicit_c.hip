@@ -118,6 +118,16 @@ int rb(rocblas_status st, const char* what)
 }
 
 // The context's rocBLAS handle (created on first use) bound to its stream.
+// y(idx) = sum of the S chunk products (chunk order), then the epilogue
+__global__ void chunk_sum_epilogue_kernel(OperatorEpilogue ep, const double* part, int S, long ncv)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ncv; i += (long)gridDim.x * blockDim.x) {
+        double y = 0.0;
+        for (int c = 0; c < S; ++c) y += part[(long)c * ncv + i];
+        ep.apply(i, 0, y);
+    }
+}
+
 rocblas_handle blas(ngw_context* ctx, int& st)
 {
     st = NGW_OK;
@@ -209,8 +219,10 @@ __global__ void weight_kernel(const double* psi, long ldp, const double* u, int 
 //   T1 = Psi_c X^T          (n_intp x nv;  X(v, c) = (s o x)(icv))
 //   t(mu) = sum_v Psi_v(mu, v) T1(mu, v)             (= C (s o x))
 //   u = M t                                           (column-split GEMV)
-//   Y^T = (Psi_v o u)^T Psi_c   (nv x nc, Y^T(v, c) = (C^T u)(icv))
-// then the shared epilogue.
+//   Y^T = (Psi_v o u)^T Psi_c   (nv x nc, Y^T(v, c) = (C^T u)(icv)),
+//         split over n_intp into S chunks (one strided-batched GEMM: the
+//         nv x nc output alone is too few tiles to fill the GPU)
+// then the chunk sum and the shared epilogue.
 int operator_f64_one(ngw_context* ctx, const OperatorArgs& a)
 {
     hipStream_t s = ctx->stream;
@@ -220,10 +232,8 @@ int operator_f64_one(ngw_context* ctx, const OperatorArgs& a)
     rocblas_handle h = blas(ctx, st);
     NGW_TRY(st);
     NGW_TRY(ctx->ex_x.ensure(sizeof(double) * (size_t)ncv));
-    NGW_TRY(ctx->ex_y.ensure(sizeof(double) * (size_t)ncv));
     NGW_TRY(ctx->ex_t.ensure(sizeof(double) * (2 * (size_t)nmu * nv + 2 * (size_t)nmu)));
     double* xs = ctx->ex_x.as<double>();
-    double* yt = ctx->ex_y.as<double>();
     double* t1 = ctx->ex_t.as<double>();  // n_intp x nv
     double* t = t1 + (size_t)nmu * nv;
     double* u = t + nmu;
@@ -247,9 +257,19 @@ int operator_f64_one(ngw_context* ctx, const OperatorArgs& a)
         ProfScope ps(ctx, KC_K3);
         hipLaunchKernelGGL(weight_kernel, dim3(std::min((nmu * (long)nv + 255) / 256, 2048L)), dim3(256), 0, s, psi,
                            ldp, u, nmu, nv, w);
-        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, nc, nmu, &one, w, nmu,
-                                 psic, (int)ldp, &zero, yt, nv), "dgemm W^T Psi_c"));
-        hipLaunchKernelGGL(explicit_epilogue_kernel, dim3(1024), dim3(256), 0, s, a.epilogue(ctx), yt, ncv, 1);
+        const int S = std::max(1, std::min(16, nmu / 256));
+        const int kc = nmu / S, rem = nmu - S * kc;
+        NGW_TRY(ctx->ex_s.ensure(sizeof(double) * (size_t)S * ncv));
+        double* part = ctx->ex_s.as<double>();
+        NGW_TRY(rb(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, nv, nc, kc,
+                                                 &one, w, nmu, (rocblas_stride)kc, psic, (int)ldp, (rocblas_stride)kc,
+                                                 &zero, part, nv, (rocblas_stride)ncv, S),
+                   "dgemm W^T Psi_c (chunks)"));
+        if (rem > 0)
+            NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, nc, rem, &one,
+                                     w + (size_t)S * kc, nmu, psic + (size_t)S * kc, (int)ldp, &one,
+                                     part + (size_t)(S - 1) * ncv, nv), "dgemm W^T Psi_c (rest)"));
+        hipLaunchKernelGGL(chunk_sum_epilogue_kernel, dim3(1024), dim3(256), 0, s, a.epilogue(ctx), part, S, ncv);
     }
     NGW_HIP(hipGetLastError());
     return NGW_OK;
@@ -263,6 +283,7 @@ void explicit_release(ngw_context* ctx)
     ctx->ex_x.release();
     ctx->ex_y.release();
     ctx->ex_t.release();
+    ctx->ex_s.release();
     ctx->explicit_c = false;
 }
 

@@ -93,7 +93,7 @@ struct ngw_context {
     // explicit pair matrix C (n_intp x ncv, ngw_set_explicit_c) in place of
     // the factored Psi_v o Psi_c; its rocBLAS handle and GEMM workspaces
     bool explicit_c = false;
-    ngw::DevBuf cexp, ex_x, ex_y, ex_t;
+    ngw::DevBuf cexp, ex_x, ex_y, ex_t, ex_s;
     ngw::DevBuf gv_part;  // chunk sums of the one-vector GEMV (gemv_wide)
     void* rb_handle = nullptr;
     // workspaces
