@@ -401,7 +401,7 @@ def config_seed(config: int, index: int = 0) -> int:
 
 
 def sigma_defaults(nv: int):
-    """NanoGW input conventions for the BASELINE workloads (Ry):
+    """NanoGW input settings of the benchmark workloads c1-c5 (Ry):
     energy_range 2 Ha, 200 points, gamma = 0.01 Ha, QP states HOMO-1..LUMO+1."""
     return {
         "deltae": 2.0 * HA_TO_RY,
