@@ -77,3 +77,63 @@ def test_qp_rows_follow_printsigma(tmp_path):
     F.write_sigma_table(path, rows)
     lines = path.read_text().splitlines()
     assert lines[:4] == F.SIGMA_HEADER and len(lines) == 6
+
+
+FMT_PROGRAM = """program fmt_check
+  implicit none
+  integer :: n, i, nn, ik, isp
+  real(8) :: v(12), h(3)
+  character(len=512) :: fin
+  call get_command_argument(1, fin)
+  open (10, file=trim(fin), status='old')
+  read (10, *) n
+  do i = 1, n
+    read (10, *) nn, v
+    write (*, '(i6,f8.3,11(1x,f11.5))') nn, v
+  end do
+  read (10, *) n
+  do i = 1, n
+    read (10, *) nn, h(1), isp, ik, h(2), h(3)
+    write (*, '(2i8,f10.3,2i4,2f10.3)') nn, nn, h(1), isp, ik, h(2), h(3)
+  end do
+end program fmt_check
+"""
+
+
+def test_formats_match_a_fortran_runtime(tmp_path):
+    """sigma_row / hmat_row against the same edit descriptors written by a
+    Fortran runtime (AMD flang), including rounding, negative values that
+    round to zero and fields that overflow to asterisks."""
+    import os
+    import subprocess
+
+    fc = os.environ.get("NGW_FC", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        pytest.skip("no Fortran compiler")
+    src, exe, data = tmp_path / "fmt.f90", tmp_path / "fmt", tmp_path / "in.txt"
+    src.write_text(FMT_PROGRAM)
+    subprocess.run([fc, "-O0", "-o", str(exe), str(src)], check=True, cwd=tmp_path, timeout=120)
+    rng = np.random.default_rng(5)
+    rows = []
+    for k in range(40):
+        vals = list(rng.normal(scale=10.0 ** rng.integers(-6, 6), size=11))
+        rows.append((int(rng.integers(1, 99999)), float(rng.choice([0.0, 1.0, 2.0, 1.9995])), vals))
+    rows.append((7, 2.0, [-0.0000049, 0.0000049, 99999.999994, -9999.999996, 123456.0, -1e-12, 0.5, -0.5,
+                          1.234565, -1.234565, 0.0]))
+    hm = [(int(rng.integers(1, 9999)), float(rng.normal(scale=30.0)), 1 + k % 2, 1 + k % 3,
+           float(rng.normal(scale=30.0)), float(rng.normal(scale=30.0))) for k in range(20)]
+    hm.append((12, -0.0004, 1, 1, 999999.9996, -99999.9996))
+    with open(data, "w") as f:
+        f.write(f"{len(rows)}\n")
+        for n, occ, vals in rows:
+            f.write(" ".join([str(n), repr(occ)] + [repr(float(v)) for v in vals]) + "\n")
+        f.write(f"{len(hm)}\n")
+        for n, eqp, isp, ik, edft, e0 in hm:
+            f.write(f"{n} {eqp!r} {isp} {ik} {edft!r} {e0!r}\n")
+    out = subprocess.run([str(exe), str(data)], capture_output=True, text=True, timeout=60, check=True).stdout
+    lines = out.splitlines()
+    assert len(lines) == len(rows) + len(hm)
+    for line, (n, occ, vals) in zip(lines, rows):
+        assert F.sigma_row(n, occ, *vals) == line, (F.sigma_row(n, occ, *vals), line)
+    for line, (n, eqp, isp, ik, edft, e0) in zip(lines[len(rows):], hm):
+        assert F.hmat_row(n, eqp, isp, ik, edft, e0) == line, (F.hmat_row(n, eqp, isp, ik, edft, e0), line)
