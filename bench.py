@@ -6,40 +6,56 @@ nen + 1 points, Sigma_c(omega) assembly and the Newton QP root -- all on the
 GPU, with the operator (Psi at the ISDF points, M, energies) resident in HBM
 when the timed region starts.
 
-Modes
-  served  (default) the QP solves are requests of a continuous-batching loop
-          (nanogw_amd.batch.serve_stream): they arrive together, are admitted
-          by HBM footprint and advance in shared grouped launches; value =
-          solves / wall time of the loop, with p50/p99 time-to-QP-energies.
-  single  one ngw_sigma_lanczos call per step (all N_b runs of the state in
-          one batched recurrence).
+Run order (default: the R = 13 bohr, nv = 100, nc = 1500, Nmu = 5000
+molecule, FP16, served mode):
+  1. build the seeded molecule (device V build), upload the operator,
+     autotune the K1/K3 tiles;
+  2. cpu_baseline: the CPU oracle on a bounded sample (about 10-20 s);
+  3. fp64_gpu: the plain FP64 GPU path on every band of one state for a few
+     Lanczos steps (every step costs the same), extrapolated to niter;
+  4. warmup: --warmup Lanczos steps of the served batch of all planned QP
+     states (untimed); their duration sets the clamp below;
+  5. timed region: --steps QP solves (distinct QP states nearest the gap:
+     HOMO, LUMO, HOMO-1, LUMO+1, ...), clamped to the configuration's QP
+     states and to the QP solves whose projected duration fits what is left
+     of --budget-s (whole run, import to JSON line); all timed states are
+     requests of one continuous-batching loop (nanogw_amd.batch.serve_stream):
+     they arrive together, are admitted by HBM footprint and advance in
+     shared grouped launches;
+  6. the JSON line is printed and flushed as soon as the timed region ends.
+`steps` in the line is the number of QP solves actually timed; `budget`
+records the request, the clamp and the total wall time.
 
-Beside the timed run the JSON line carries
-  fp64_gpu      the plain FP64 GPU path on the same state's full batch for a
-                few Lanczos steps (every step costs the same), extrapolated
-                to QP solves/s, and the speed-up over it;
+Fields beside the contract's:
   qp_error      |E_qp(mixed) - E_qp(FP64)| per timed state, against the FP64
                 GPU path's QP energies of the same seeded molecule stored in
-                tests/golden/<config>_fp64_qp.json (written by --write-golden
-                with --precision f64), after checking that the molecule's
-                fingerprint matches;
-  roofline      the dominant GEMM's algorithmic FLOP / its summed HIP-event
-                time over the timed region, against the MI355X dense peak of
-                its arithmetic; traffic = its HBM bytes per launch from the
-                copy-calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
-                profiles/pmc_traffic_c3.md (default c3 served run only);
+                tests/golden/<config>_fp64_qp.json (--write-golden with
+                --precision f64), after checking the molecule's fingerprint;
+  roofline      the dominant GEMM's algorithmic FLOP per launch / its average
+                HIP-event launch time over the timed region, against the
+                MI355X dense peak of its arithmetic; traffic = HBM bytes per
+                launch from the copy-calibrated rocprofv3 FETCH_SIZE +
+                WRITE_SIZE summary of bench.py itself (profiles/pmc_traffic_*.md)
+                at the same kernel, precision, tile configuration and vectors
+                per launch;
   cpu_baseline  the CPU oracle on a bounded sample, extrapolated.
+--profile-steps N replaces niter by N in the timed solves, so a profiler run
+of the same workload ends in seconds; such a line is marked "profile_run" and
+is not a headline.
 Only one GPU is supported.
 """
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
 import sys
 import time
 
-import numpy as np
+T_START = time.perf_counter()
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -47,10 +63,17 @@ sys.path.insert(0, ROOT)
 PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15, "bf16x3": 2.5e15, "f16": 2.5e15}
 RYD = 13.60569193
 GOLDEN = os.path.join(ROOT, "tests", "golden")
+# The driver kills a bench run after 600 s; the default budget keeps a margin.
+DEFAULT_BUDGET_S = 450.0
+DRIVER_LIMIT_S = 600.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def since_start():
+    return time.perf_counter() - T_START
 
 
 def build_workload(cfg, ctx, seed_index=0):
@@ -67,11 +90,33 @@ def build_workload(cfg, ctx, seed_index=0):
     mol.ridge = ridge
     log(f"[bench] {cfg}: ngrid={mol.ngrid} nfft={mol.nfft} nv={nv} nc={nc} nmu={nmu} "
         f"built in {time.time() - t:.1f}s")
-    # QP states nearest the gap first: HOMO, LUMO, HOMO-1, LUMO+1, ...
+    return mol, niter, gap_order(nv, nqp)
+
+
+def gap_order(nv, nqp):
+    """QP states nearest the gap first: HOMO, LUMO, HOMO-1, LUMO+1, ... (0-based)."""
     order = []
     for k in range(nqp // 2):
         order += [nv - 1 - k, nv + k]
-    return mol, niter, order
+    return order
+
+
+def plan_solves(requested, n_states, niter, step_s_at_plan, n_plan, remaining_s, margin=1.10):
+    """Number of QP solves to time.  `requested` is clamped to the
+    configuration's n_states, then to the largest k whose projected duration
+    niter * step_s_at_plan * k / n_plan * margin fits remaining_s (the batch's
+    Lanczos step time scales with its column count).  At least one solve is
+    timed.  Returns (k, reason)."""
+    k = max(1, min(requested, n_states))
+    reason = "requested" if k == requested else f"configuration has {n_states} QP states"
+    if step_s_at_plan is None or n_plan <= 0:
+        return k, reason
+    per_solve = niter * step_s_at_plan / n_plan * margin
+    fit = int(remaining_s // per_solve) if per_solve > 0 else k
+    if fit < k:
+        return max(1, fit), (f"wall budget: {remaining_s:.0f} s left, about {per_solve:.0f} s per "
+                             f"batched QP solve")
+    return k, reason
 
 
 def fingerprint(mol):
@@ -107,58 +152,84 @@ def cpu_baseline(mol, poly, niter, nb_sample=8, target_s=15.0):
         dt = time.time() - t
     per_qp = dt * (niter / steps) * (mol.nb / nb_sample)
     return {"value": 1.0 / per_qp, "unit": "QP solves/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"oracle BLOCK Lanczos, {nb_sample} bands x {steps} of {niter} steps "
-                      f"(npoly={len(poly) - 1}), {dt:.1f}s, extrapolated x{niter / steps:.1f} steps "
-                      f"x{mol.nb / nb_sample:.1f} bands per QP solve"}
+            "sample": f"oracle BLOCK Lanczos (explicit C rebuilt per matvec, as apply_matvec_BLOCK.F90), "
+                      f"{nb_sample} of the state's {mol.nb} bands x {steps} of {niter} Lanczos steps "
+                      f"(npoly={len(poly) - 1}, {12 * steps} matvecs on {nb_sample} vectors) in {dt:.1f} s; "
+                      f"every step and band block costs the same, so one QP solve = "
+                      f"x{niter / steps:.0f} steps x{mol.nb / nb_sample:.0f} band blocks"}
 
 
 def fp64_sample(ctx, mol, poly, niter, steps, chunk=200):
     """Plain FP64 GPU path for one QP state, in its fastest form: the N_b
     Lanczos runs in chunks of `chunk` columns (the FP64 GEMMs run faster per
-    column on 200-column batches than on all 1600 at once).  One chunk is
-    timed for `steps` Lanczos steps; every step and every chunk does the same
-    work, so QP solves/s is extrapolated linearly (x niter/steps x N_b/chunk)."""
+    column on 200-column batches than on all 1600 at once).  Every chunk of
+    the state is run for 1 and for 1 + `steps` Lanczos steps; the difference
+    is the time of `steps` steps of all N_b columns without the fixed
+    per-call work (upload, continued fraction), and QP solves/s follows by
+    x niter / steps (every Lanczos step does the same work)."""
     nv, nc, nb = mol.nv, mol.nc, mol.nb
-    cols = min(chunk, nb)
     rng = np.random.default_rng(5)
-    v0 = rng.normal(size=(nv * nc, cols))
-    vn = np.linalg.norm(v0, axis=0)
-    zz = np.zeros((2, cols), np.complex128) + 0.02j
     ctx.set_precision("f64")
-    ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, 1, poly)  # allocate
-    ctx.synchronize()
-    t = time.perf_counter()
-    ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, steps, poly)
-    dt = time.perf_counter() - t
-    per_qp = dt * (niter / steps) * (nb / cols)
+    t_one = t_many = 0.0
+    for lo in range(0, nb, chunk):
+        cols = min(chunk, nb - lo)
+        v0 = rng.normal(size=(nv * nc, cols))
+        vn = np.linalg.norm(v0, axis=0)
+        zz = np.zeros((2, cols), np.complex128) + 0.02j
+        if lo == 0:
+            ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, 1, poly)  # allocate
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, 1, poly)
+        t_one += time.perf_counter() - t
+        t = time.perf_counter()
+        ctx.lanczos_spectra_isdf_BLOCK(v0, vn, zz, 1 + steps, poly)
+        t_many += time.perf_counter() - t
+    per_step = max(t_many - t_one, 1e-9) / steps
+    per_qp = per_step * niter
     return {"qp_solves_per_s": 1.0 / per_qp, "timed_lanczos_steps": steps, "of_niter": niter,
-            "seconds": dt, "columns": cols,
-            "note": f"FP64 GPU path (ngw_lanczos_spectra_isdf_block) on {cols} of the state's {nb} columns for "
-                    f"{steps} Lanczos steps, extrapolated to niter and N_b"}
+            "seconds_per_lanczos_step": per_step, "columns_per_call": chunk, "columns": nb,
+            "note": f"FP64 GPU path (ngw_lanczos_spectra_isdf_block) on all {nb} columns of one QP state in "
+                    f"{chunk}-column calls, (1 + {steps}) - 1 Lanczos steps per call, x{niter / steps:.0f} "
+                    f"to niter = {niter}"}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c3.md")
+def traffic_files():
+    return sorted(glob.glob(os.path.join(ROOT, "profiles", "pmc_traffic_*.md")))
 
 
-def measured_traffic(kernel, precision, cfg):
-    """HBM bytes per launch of a GEMM kernel at the served c3 launch shape
-    (3200 vectors) in tile configuration `cfg`, from the copy-calibrated
-    rocprofv3 FETCH_SIZE + WRITE_SIZE summary committed under profiles/
-    (lines 'traffic <kernel> <precision> cfg<index> <bytes>'); None when not
-    measured."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            for line in f:
-                p = line.split()
-                if (len(p) == 5 and p[0] == "traffic" and p[1] == kernel and p[2] == precision
-                        and p[3] == f"cfg{cfg}"):
-                    return float(p[4])
-    except OSError:
-        pass
-    return None
+def measured_traffic(kernel, precision, cfg, nvec, files=None):
+    """HBM bytes per launch of an operator GEMM kernel from the copy-calibrated
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summaries committed under profiles/
+    (lines 'traffic <kernel> <precision> cfg<index> nvec<vectors> <bytes>').
+    An exact (kernel, precision, cfg, nvec) match is returned as measured;
+    otherwise the bytes of the same kernel, precision and cfg at the nearest
+    measured vector count are scaled by the vector ratio (the grouped launch
+    of k requests moves k times the bytes of one).  Returns (bytes, source)
+    or (None, None)."""
+    best = None
+    for path in files if files is not None else traffic_files():
+        try:
+            lines = open(path).read().splitlines()
+        except OSError:
+            continue
+        for line in lines:
+            p = line.split()
+            if not (len(p) == 6 and p[0] == "traffic" and p[1] == kernel and p[2] == precision
+                    and p[3] == f"cfg{cfg}" and p[4].startswith("nvec")):
+                continue
+            nv_m, val = int(p[4][4:]), float(p[5])
+            name = os.path.relpath(path, ROOT)
+            if nv_m == nvec:
+                return val, f"measured ({name}, {nvec} vectors per launch)"
+            if best is None or abs(nv_m - nvec) < abs(best[0] - nvec):
+                best = (nv_m, val, name)
+    if best is None:
+        return None, None
+    return best[1] * nvec / best[0], f"scaled x{nvec / best[0]:.3f} from {best[0]} vectors per launch ({best[2]})"
 
 
-def roofline_from_profile(prof, flops, precision, traffic_shape_ok=False, tiles=None):
+def roofline_from_profile(prof, flops, precision, tiles=None, nvec=None):
     dom = max(flops, key=lambda k: prof[k][0])
     ms, cnt = prof[dom]
     if cnt == 0 or ms <= 0:
@@ -166,10 +237,13 @@ def roofline_from_profile(prof, flops, precision, traffic_shape_ok=False, tiles=
     ach = flops[dom] / (ms / 1000.0)
     peak = PEAK[precision]
     cfg = None if tiles is None else {"K1": tiles[0], "K3": tiles[1]}.get(dom)
-    traffic = measured_traffic(dom, precision, cfg) if traffic_shape_ok and cfg is not None else None
+    traffic, source = (None, None)
+    if cfg is not None and nvec:
+        traffic, source = measured_traffic(dom, precision, cfg, nvec)
     return {"bound": "mfma", "kernel": dom, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": ach / peak, "traffic": traffic, "avg_launch_us": ms * 1000.0 / cnt, "launches": cnt,
-            "flop_per_launch": flops[dom] / cnt}
+            "frac": ach / peak, "traffic": traffic, "traffic_source": source,
+            "avg_launch_us": ms * 1000.0 / cnt, "launches": cnt, "flop_per_launch": flops[dom] / cnt,
+            "vectors_per_launch": nvec, "tile_config": cfg}
 
 
 def run_batched(args):
@@ -321,8 +395,14 @@ def run_stream(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2, help="QP solves timed")
-    ap.add_argument("--warmup", type=int, default=0, help="untimed QP solves before the timed ones")
+    ap.add_argument("--steps", type=int, default=8,
+                    help="QP solves to time (distinct states nearest the gap), clamped to the "
+                         "configuration's QP states and to --budget-s")
+    ap.add_argument("--warmup", type=int, default=2,
+                    help="untimed Lanczos steps of the served batch before the timed region")
+    ap.add_argument("--budget-s", type=float, default=DEFAULT_BUDGET_S,
+                    help="wall budget of the whole run (import to JSON line); the timed QP solves are "
+                         "clamped to what fits")
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--molecules", type=int, default=64, help="c4 / c5: number of (catalog) molecules")
     ap.add_argument("--rate", type=float, default=20.0, help="c5: arrivals per second")
@@ -336,8 +416,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile configurations instead of timing them on this shape")
-    ap.add_argument("--fp64-steps", type=int, default=2,
-                    help="Lanczos steps of the plain FP64 GPU path timed for the speed-up")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="Lanczos steps of the plain FP64 GPU path timed for the speed-up (0: skip)")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="profiler runs: Lanczos steps per timed solve instead of niter (not a headline)")
     ap.add_argument("--band-chunk", type=int, default=0,
                     help="single mode: sum Sigma_c over band ranges of this many bands "
                          "(ngw_sigma_lanczos_bands), logging each range")
@@ -365,25 +447,32 @@ def main():
     from nanogw_amd.lanczos import IsdfContext, lanczos_polynomial
 
     ctx = IsdfContext(0)
-    mol, niter, order = build_workload(args.config, ctx)
-    poly = lanczos_polynomial(mol.e1, mol.nv, 12)
-    ctx.set_isdf(mol.psi_intp, mol.mmtrx, mol.e1, mol.nv)
-    ctx.set_precision(args.precision)
-    sd = synth.sigma_defaults(mol.nv)
-    nen, deltae, ecuts = sd["nen"], sd["deltae"], sd["ecuts"]
-    n_all = args.warmup + args.steps
+    mol, niter_cfg, order = build_workload(args.config, ctx)
+    n_states_cfg = len(order)
     if args.qp_states:
         order = [int(v) for v in args.qp_states.split(",")]
-    states = [order[i % len(order)] for i in range(n_all)]
-    sigx_all = synth.exchange_diag(mol.psi_intp, mol.mmtrx, mol.nv, order)
-    sigx_of = dict(zip(order, sigx_all))
+    niter = args.profile_steps if args.profile_steps > 0 else niter_cfg
+    poly = lanczos_polynomial(mol.e1, mol.nv, 12)
+    ctx.set_isdf(mol.psi_intp, mol.mmtrx, mol.e1, mol.nv)
+    sd = synth.sigma_defaults(mol.nv)
+    nen, deltae, ecuts = sd["nen"], sd["deltae"], sd["ecuts"]
     nv, nc, nmu, nb = mol.nv, mol.nc, mol.n_intp, mol.nb
     npoly = len(poly) - 1
+    sigx_all = synth.exchange_diag(mol.psi_intp, mol.mmtrx, mol.nv, order)
+    sigx_of = dict(zip(order, sigx_all))
 
-    def newton(sig, sel):
-        e = mol.e1[sel] * RYD
-        sx = np.array([sigx_of[m] for m in sel]) * RYD
-        return ctx.qp_newton(sig * RYD, deltae * RYD, e, e, sx, np.zeros(len(sel)))
+    # baselines first, so that the line can be printed the moment the timed region ends
+    cpu = None
+    if not args.no_cpu_baseline:
+        t = time.time()
+        cpu = cpu_baseline(mol, poly, niter_cfg)
+        log(f"[bench] cpu baseline (oracle sample): {time.time() - t:.1f}s")
+    fp64_ref = None
+    if args.precision != "f64" and args.fp64_steps > 0:
+        t = time.time()
+        fp64_ref = fp64_sample(ctx, mol, poly, niter_cfg, args.fp64_steps)
+        log(f"[bench] fp64 GPU sample: {time.time() - t:.1f}s")
+    ctx.set_precision(args.precision)
 
     tiles = None
     if args.precision != "f64" and not args.no_autotune:
@@ -392,48 +481,68 @@ def main():
         log(f"[bench] autotune on {nb} vectors: K1 config {tiles[0]}, K3 config {tiles[1]} "
             f"({time.time() - t:.1f}s)")
 
+    def newton(sig, sel):
+        e = mol.e1[sel] * RYD
+        sx = np.array([sigx_of[m] for m in sel]) * RYD
+        return ctx.qp_newton(sig * RYD, deltae * RYD, e, e, sx, np.zeros(len(sel)))
+
+    plan = order[:max(1, min(args.steps, len(order)))]
     results = {}
     lat = None
     serve_stats = None
+    step_s = None
     if args.mode == "served":
         batch = GwBatch(ctx)
         sid = batch.add_system(mol.psi_intp, mol.mmtrx, mol.e1, mol.nv)
-        budget = int(0.85 * torch.cuda.mem_get_info(0)[0])
+        budget_bytes = int(0.85 * torch.cuda.mem_get_info(0)[0])
         sigx_fn = [lambda qp: np.array([sigx_of[m] for m in qp])]
-        if args.warmup:
-            t = time.time()
-            wreq = [{"t": 0.0, "mol": 0, "qp": [m]} for m in states[:args.warmup]]
-            serve_stream(ctx, [mol], wreq, niter, nen, deltae, ecuts, budget, sigx=sigx_fn,
-                         use_graph=args.graph, batch=batch, sids=[sid], polys=[poly])
-            log(f"[bench] warmup ({args.warmup} solves): {time.time() - t:.1f}s")
-        reqs = [{"t": 0.0, "mol": 0, "qp": [m]} for m in states[args.warmup:]]
-        ctx.profile_enable(not args.graph)
-        ctx.profile_reset()
-        torch.cuda.synchronize()
+        # warmup: every planned state as one request of the batch; the first
+        # step builds the P vectors, the next --warmup steps are timed for the clamp
+        w = max(1, args.warmup)
+        t = time.time()
+        rids = [batch.submit(sid, [m], nen, deltae, ecuts, w + 1, poly) for m in plan]
+        batch.step(1, args.graph)
         ctx.synchronize()
-        t0 = time.perf_counter()
-        recs, serve_stats = serve_stream(ctx, [mol], reqs, niter, nen, deltae, ecuts, budget, sigx=sigx_fn,
-                                         use_graph=args.graph, batch=batch, sids=[sid], polys=[poly],
-                                         log=log)
+        t1 = time.perf_counter()
+        batch.step(w, args.graph)
+        ctx.synchronize()
+        step_s = (time.perf_counter() - t1) / w
+        for rid in rids:
+            batch.release(rid)
+        log(f"[bench] warmup: {w} + 1 Lanczos steps of {len(plan)} states x {nb} columns in "
+            f"{time.time() - t:.1f}s ({step_s:.3f} s per step)")
+    else:
+        w = max(1, args.warmup)
+        t = time.time()
+        ctx.calculate_sigma_lanczos([plan[0]], nen, deltae, ecuts, 1, poly)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.calculate_sigma_lanczos([plan[0]], nen, deltae, ecuts, 1 + w, poly)
+        ctx.synchronize()
+        step_s = (time.perf_counter() - t1) / (1 + w) * len(plan)  # per state, run one after another
+        log(f"[bench] warmup: {w} + 1 Lanczos steps of one state in {time.time() - t:.1f}s")
+    remaining = args.budget_s - since_start() - 15.0  # Newton, JSON and teardown
+    k, clamp_reason = plan_solves(args.steps, len(order), niter, step_s, len(plan), remaining)
+    states = plan[:k]
+    log(f"[bench] timing {k} QP solve(s) {states} ({clamp_reason}); {since_start():.0f}s since start")
+
+    ctx.profile_enable(not args.graph)
+    ctx.profile_reset()
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    if args.mode == "served":
+        reqs = [{"t": 0.0, "mol": 0, "qp": [m]} for m in states]
+        recs, serve_stats = serve_stream(ctx, [mol], reqs, niter, nen, deltae, ecuts, budget_bytes, sigx=sigx_fn,
+                                         use_graph=args.graph, batch=batch, sids=[sid], polys=[poly], log=log)
         ctx.synchronize()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         lat = latency_stats(recs)
         for r in recs:
             results[r["qp"][0]] = float(r["eqp"][0])
-        batch.close()
-        per_req_cols = nb
-        cols_total = per_req_cols * args.steps
+        nvec_launch = nb * k
     else:
-        for i in range(args.warmup):
-            t = time.time()
-            newton(ctx.calculate_sigma_lanczos([states[i]], nen, deltae, ecuts, niter, poly), [states[i]])
-            log(f"[bench] warmup step {i}: {time.time() - t:.2f}s")
-        ctx.profile_enable(True)
-        ctx.profile_reset()
-        torch.cuda.synchronize()
-        ctx.synchronize()
-        t0 = time.perf_counter()
         def sigma_of(m):
             if args.band_chunk <= 0:
                 return ctx.calculate_sigma_lanczos([m], nen, deltae, ecuts, niter, poly)
@@ -445,74 +554,48 @@ def main():
                 log(f"[bench] state {m}: bands {lo}..{hi - 1} done in {time.time() - t:.1f}s")
             return sig
 
-        for i in range(args.steps):
-            m = states[args.warmup + i]
+        for i, m in enumerate(states):
             r = newton(sigma_of(m), [m])
             results[m] = float(r["eqp"][0])
-            log(f"[bench] step {i}: state {m} E_qp = {r['eqp'][0]:.6f} eV")
+            log(f"[bench] solve {i}: state {m} E_qp = {r['eqp'][0]:.6f} eV")
         ctx.synchronize()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        cols_total = nb * args.steps
+        nvec_launch = nb if args.band_chunk <= 0 else min(nb, args.band_chunk)
     prof = ctx.profile_read()
     ctx.profile_enable(False)
-    solves = args.steps
-    value = solves / elapsed
-    ms_per_step = 1000.0 * elapsed / args.steps
+    value = k / elapsed
 
     # algorithmic FLOP of each GEMM class over the timed region
-    mv_total = float(cols_total) * niter * npoly
+    mv_total = float(nb) * k * niter * npoly
     flops = {"K1": 2.0 * nmu * nc * nv * mv_total, "K2": 2.0 * nmu * nmu * mv_total,
              "K3": 2.0 * nc * nmu * nv * mv_total}
-    # the committed PMC traffic was measured at the launch shape of the default
-    # served c3 run: both timed states (2 x 1600 vectors) in every launch
-    shape_ok = args.config == "c3" and args.mode == "served" and args.steps == 2
-    roof = roofline_from_profile(prof, flops, args.precision, shape_ok, tiles) if prof["K1"][1] else None
+    roof = roofline_from_profile(prof, flops, args.precision, tiles, nvec_launch) if prof["K1"][1] else None
     f_mv = 4.0 * nmu * nv * nc + 2.0 * nmu * nmu
 
     # QP error against the stored FP64 reference of this molecule
     gpath = os.path.join(GOLDEN, f"{args.config}_fp64_qp.json")
     fp = fingerprint(mol)
     qp_error = None
-    if os.path.exists(gpath):
+    if os.path.exists(gpath) and args.profile_steps <= 0:
         ref = json.load(open(gpath))
         if same_molecule(fp, ref["fingerprint"]):
             errs = {str(m): 1000.0 * abs(results[m] - ref["eqp_ev"][str(m)]) for m in results
                     if str(m) in ref["eqp_ev"]}
             qp_error = {"meV": errs, "max_meV": max(errs.values()) if errs else None,
-                        "reference": os.path.relpath(gpath, ROOT)}
+                        "reference": os.path.relpath(gpath, ROOT), "made_by": ref.get("made_by")}
         else:
             qp_error = {"max_meV": None, "note": "molecule fingerprint differs from the stored FP64 reference"}
-    if args.write_golden and args.precision == "f64":
-        old = json.load(open(gpath)) if os.path.exists(gpath) else {"eqp_ev": {}}
-        if "fingerprint" in old and not same_molecule(fp, old["fingerprint"]):
-            old = {"eqp_ev": {}}
-        old["fingerprint"] = fp
-        old["eqp_ev"].update({str(m): e for m, e in results.items()})
-        old["config"] = args.config
-        old["niter"], old["nen"], old["npoly"] = niter, nen, npoly
-        old["made_by"] = (f"python bench.py --config {args.config} --precision f64 --mode {args.mode} "
-                          f"--band-chunk {args.band_chunk} --write-golden (one QP state per run)")
-        json.dump(old, open(gpath, "w"), indent=1)
-        log(f"[bench] wrote {gpath}")
-
-    fp64_ref = None
-    if args.precision != "f64" and args.fp64_steps > 0:
-        fp64_ref = fp64_sample(ctx, mol, poly, niter, args.fp64_steps)
-        fp64_ref["speedup"] = value / fp64_ref["qp_solves_per_s"]
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("[bench] cpu baseline (oracle sample)...")
-        cpu = cpu_baseline(mol, poly, niter)
+    total_s = since_start()
     out = {
         "metric": "QP solves/s (ISDF-Lanczos full-frequency GW) + p50/p99 time-to-QP-energies, "
                   "vs plain FP64 GPU path at <=1 meV QP error",
         "value": value,
         "unit": "QP solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "steps": k,
+        "warmup": w,
+        "ms_per_step": 1000.0 * elapsed / k,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -525,21 +608,42 @@ def main():
                                 "loop (all N_b bands of every state in each launch)" if args.mode == "served"
                                 else "single mode: one ngw_sigma_lanczos call per QP state") +
                                (", hipGraph replay" if args.graph else ""),
-                   "qp_states": states[args.warmup:], "parallelism": "single GPU",
-                   "tile_configs": tiles},
+                   "qp_states": states, "parallelism": "single GPU", "tile_configs": tiles},
+        "budget": {"wall_budget_s": args.budget_s, "steps_requested": args.steps, "steps_timed": k,
+                   "clamp": clamp_reason, "warmup_lanczos_steps": w,
+                   "warmup_s_per_lanczos_step": step_s, "timed_s": elapsed, "total_s": total_s,
+                   "driver_limit_s": DRIVER_LIMIT_S, "fits_in_driver_run": total_s < DRIVER_LIMIT_S},
         "eqp_ev": {str(m): e for m, e in results.items()},
         "latency": lat,
         "serve": serve_stats,
         "qp_error": qp_error,
         "fp64_gpu": fp64_ref,
+        "speedup_vs_fp64_gpu": (value / fp64_ref["qp_solves_per_s"]) if fp64_ref else None,
         "matvecs_per_s": mv_total / elapsed,
         "gflops": f_mv * mv_total / elapsed / 1e9,
         "operator_frac_of_peak": f_mv * mv_total / elapsed / PEAK[args.precision],
         "roofline": roof,
-        "kernel_ms": {k: v[0] for k, v in prof.items()},
+        "kernel_ms": {kk: v[0] for kk, v in prof.items()},
         "cpu_baseline": cpu,
     }
+    if args.profile_steps > 0:
+        out["profile_run"] = f"niter replaced by {niter} Lanczos steps per solve; not a headline"
     print(json.dumps(out), flush=True)
+
+    if args.write_golden and args.precision == "f64" and args.profile_steps <= 0:
+        old = json.load(open(gpath)) if os.path.exists(gpath) else {"eqp_ev": {}}
+        if "fingerprint" in old and not same_molecule(fp, old["fingerprint"]):
+            old = {"eqp_ev": {}}
+        old["fingerprint"] = fp
+        old["eqp_ev"].update({str(m): e for m, e in results.items()})
+        old["config"] = args.config
+        old["niter"], old["nen"], old["npoly"] = niter, nen, npoly
+        old["made_by"] = (f"python bench.py --config {args.config} --precision f64 --mode {args.mode} "
+                          f"--band-chunk {args.band_chunk} --write-golden (one QP state per run)")
+        json.dump(old, open(gpath, "w"), indent=1)
+        log(f"[bench] wrote {gpath}")
+    if args.mode == "served":
+        batch.close()
     ctx.close()
 
 
