@@ -23,9 +23,11 @@
  *   orc_polynomial_matvec_block : Z p(Lambda) Z^T from numpy eigh.
  *   orc_sigma_lanczos         : sum-over-states with lambda_s = p(Omega_s^2)
  *                               at niter = ncv (Lanczos exact limit).
- *   orc_spline_evaluate / orc_newtonr / orc_qp_solve : exactness on cubic
- *                               tables away from the ends, one-step Newton on
- *                               linear f(E), hand-worked spline cases.
+ *   orc_spline_evaluate / orc_newtonr / orc_qp_solve : a cubic table is
+ *                               reproduced to rounding away from the ends,
+ *                               O(h^4) convergence on smooth tables, one-step
+ *                               Newton on linear f(E), the closed-form root of
+ *                               a linear Sigma_c.
  *   orc_matvec_explicit / orc_polynomial_matvec_explicit : with Cmtrx
  *                               built from the orbital pair products they
  *                               equal the BLOCK restatements column by
@@ -485,10 +487,13 @@ void orc_pvectors(int nmu, int nv, int nc, const double *psi, long ldpsi, const 
  *   deltae    energy_range (Ry);  ecuts  dynamic_energy_resolution (Ry)
  *   sigma(nen, nqp) complex, Ry.
  */
-void orc_sigma_lanczos(int nmu, int nv, int nc, const double *psi, long ldpsi,
-                       const double *mmat, long ldm, const double *e1, int nqp, const int *qp,
-                       int nen, double deltae, double ecuts, int niter, const double *poly,
-                       int npoly, int jnblock, int tda, double complex *sigma)
+/* The band loop restricted to Green's-function bands band_lo <= n < band_hi
+ * (0-based); Sigma_c is a sum over bands, so ranges add up to the full
+ * result.  orc_sigma_lanczos below is the full range. */
+void orc_sigma_lanczos_bands(int nmu, int nv, int nc, const double *psi, long ldpsi,
+                             const double *mmat, long ldm, const double *e1, int nqp, const int *qp,
+                             int band_lo, int band_hi, int nen, double deltae, double ecuts, int niter,
+                             const double *poly, int npoly, int jnblock, int tda, double complex *sigma)
 {
     int nb = nv + nc;
     long ncv = (long)nv * nc;
@@ -502,11 +507,13 @@ void orc_sigma_lanczos(int nmu, int nv, int nc, const double *psi, long ldpsi,
     /* band blocks, occupied first (:210-245) */
     int nblks = 0;
     int (*blks)[3] = malloc(sizeof(int[3]) * (nb / (jnblock > 0 ? jnblock : 1) + 4));
-    for (int s = 0; s < nv; s += jnblock) {
-        blks[nblks][0] = s; blks[nblks][1] = (s + jnblock < nv ? s + jnblock : nv) - 1; blks[nblks][2] = 1; ++nblks;
+    int vlo = band_lo > 0 ? band_lo : 0, vhi = band_hi < nv ? band_hi : nv;
+    int clo = band_lo > nv ? band_lo : nv, chi = band_hi < nb ? band_hi : nb;
+    for (int s = vlo; s < vhi; s += jnblock) {
+        blks[nblks][0] = s; blks[nblks][1] = (s + jnblock < vhi ? s + jnblock : vhi) - 1; blks[nblks][2] = 1; ++nblks;
     }
-    for (int s = nv; s < nb; s += jnblock) {
-        blks[nblks][0] = s; blks[nblks][1] = (s + jnblock < nb ? s + jnblock : nb) - 1; blks[nblks][2] = 0; ++nblks;
+    for (int s = clo; s < chi; s += jnblock) {
+        blks[nblks][0] = s; blks[nblks][1] = (s + jnblock < chi ? s + jnblock : chi) - 1; blks[nblks][2] = 0; ++nblks;
     }
     double *pvec = (double *)malloc(sizeof(double) * ncv * jnblock);
     double *pnorm2 = (double *)malloc(sizeof(double) * jnblock);
@@ -554,6 +561,15 @@ void orc_sigma_lanczos(int nmu, int nv, int nc, const double *psi, long ldpsi,
         }
     }
     free(sqrtR); free(cm); free(blks); free(pvec); free(pnorm2); free(pnorm); free(ezz); free(sct);
+}
+
+void orc_sigma_lanczos(int nmu, int nv, int nc, const double *psi, long ldpsi,
+                       const double *mmat, long ldm, const double *e1, int nqp, const int *qp,
+                       int nen, double deltae, double ecuts, int niter, const double *poly,
+                       int npoly, int jnblock, int tda, double complex *sigma)
+{
+    orc_sigma_lanczos_bands(nmu, nv, nc, psi, ldpsi, mmat, ldm, e1, nqp, qp, 0, nv + nc, nen, deltae, ecuts,
+                            niter, poly, npoly, jnblock, tda, sigma);
 }
 
 /* ------------------------------------------------------------------ */

@@ -15,9 +15,12 @@ Pinning (tests/test_oracle_*.py):
     moment identity;
   * operator, Horner, Sigma_c: dense closed forms built independently with
     numpy eigh, sum over states;
-  * poisson: analytic Coulomb energy of Gaussian charges (erf form);
-    isdf_kernel: symmetry, positive semi-definiteness, Nmu = Ngrid limit;
-  * spline/Newton: exactness on cubic tables, one-step Newton on linear f;
+  * poisson: analytic potential of a Gaussian charge (erf form);
+    isdf_kernel: absolute scale (ntrans/h^3, e^2 = 2) from the pair Coulomb
+    energies of Gaussian zeta columns at two grid spacings and two ntrans,
+    symmetry, positive semi-definiteness;
+  * spline/Newton: exactness on a cubic table away from the ends (the end
+    slopes are finite differences), one-step Newton on linear f;
   * polyfit: numpy.linalg.lstsq on well-conditioned cases.  The npoly = 12
     LAPACK dgesvx coefficients are parity unpinned; they are pinned inputs.
   * cvt_lattice: a brute-force Lloyd step (full distance matrix, weighted
@@ -78,6 +81,9 @@ def lib():
         L.orc_sigma_lanczos.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
                                         _dp, C.c_int, _ip, C.c_int, C.c_double, C.c_double,
                                         C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_sigma_lanczos_bands.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
+                                              _dp, C.c_int, _ip, C.c_int, C.c_int, C.c_int, C.c_double,
+                                              C.c_double, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_spline_evaluate.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
         L.orc_newtonr.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, _dp]
         L.orc_newtonr.restype = C.c_int
@@ -224,15 +230,22 @@ def pvectors(psi, mmat, sqrtR, m, n0, blksz, nv, nc, tda=False):
     return p, pn2
 
 
-def sigma_lanczos(psi, mmat, e1, nv, nc, qp, nen, deltae, ecuts, niter, poly, jnblock, tda=False):
-    """Sigma_c(omega) (nen, nqp) complex in Ry for 0-based QP band indices `qp`."""
+def sigma_lanczos(psi, mmat, e1, nv, nc, qp, nen, deltae, ecuts, niter, poly, jnblock, tda=False, bands=None):
+    """Sigma_c(omega) (nen, nqp) complex in Ry for 0-based QP band indices `qp`;
+    bands = (lo, hi) keeps only Green's-function bands lo <= n < hi."""
     psi, mmat, e1, poly = _f64(psi), _f64(mmat), _f64(e1), _f64(poly)
     qp = np.ascontiguousarray(qp, np.int32)
     out = np.zeros((nen, len(qp)), np.complex128, order="F")
     nmu = psi.shape[0]
-    lib().orc_sigma_lanczos(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0], _p(e1),
-                            len(qp), qp.ctypes.data_as(_ip), nen, float(deltae), float(ecuts),
-                            int(niter), _p(poly), len(poly) - 1, int(jnblock), int(tda), _p(out))
+    if bands is None:
+        lib().orc_sigma_lanczos(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0], _p(e1),
+                                len(qp), qp.ctypes.data_as(_ip), nen, float(deltae), float(ecuts),
+                                int(niter), _p(poly), len(poly) - 1, int(jnblock), int(tda), _p(out))
+    else:
+        lib().orc_sigma_lanczos_bands(nmu, nv, nc, _p(psi), psi.shape[0], _p(mmat), mmat.shape[0], _p(e1),
+                                      len(qp), qp.ctypes.data_as(_ip), int(bands[0]), int(bands[1]), nen,
+                                      float(deltae), float(ecuts), int(niter), _p(poly), len(poly) - 1,
+                                      int(jnblock), int(tda), _p(out))
     return out
 
 

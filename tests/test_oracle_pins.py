@@ -104,8 +104,9 @@ def test_horner_is_p_of_h(O, tiny):
     assert rel_err(y, pH @ x) < 1e-9
 
 
-def _sigma_sum_over_states(mol, qp, nen, deltae, ecuts, poly, tda=False):
-    """Sigma_c from dense eigh: S(z) = sum_s |<P|Z_s>|^2 / (z - p(Omega_s^2))."""
+def _sigma_sum_over_states(mol, qp, nen, deltae, ecuts, poly, tda=False, bands=None):
+    """Sigma_c from dense eigh: S(z) = sum_s |<P|Z_s>|^2 / (z - p(Omega_s^2));
+    bands = (lo, hi) keeps Green's-function bands lo <= n < hi."""
     H, s, C = _dense_h(mol, tda)
     w, Z = np.linalg.eigh(H)
     lam = np.polyval(poly, w)
@@ -113,7 +114,7 @@ def _sigma_sum_over_states(mol, qp, nen, deltae, ecuts, poly, tda=False):
     out = np.zeros((nen, len(qp)), complex)
     for iq, m in enumerate(qp):
         z = deltae / (nen - 1) * np.arange(nen) + mol.e1[m] - deltae / 2
-        for n in range(nb):
+        for n in range(nb) if bands is None else range(*bands):
             c = mol.psi_intp[:, m] * mol.psi_intp[:, n]
             P = C.T @ (mol.mmtrx @ c)
             if not tda:
@@ -142,6 +143,22 @@ def test_sigma_equals_sum_over_states(O, tiny, tda):
                           ncv, poly, 3, tda=tda)
     ref = _sigma_sum_over_states(tiny, qp, 31, 3.0, 0.05, poly, tda)
     assert rel_err(sig, ref) < 1e-7
+
+
+@pytest.mark.parametrize("bands", [(0, 2), (1, 5), (2, 7)])
+def test_sigma_band_range_equals_sum_over_states(O, tiny, bands):
+    """The band-range restriction (a multi-device driver's unit of work):
+    Sigma_c over Green's-function bands lo <= n < hi equals the sum over
+    Casida states restricted to those bands, and ranges add up to the whole."""
+    poly = np.array([-0.004, 0.09, 0.6, 0.35])
+    qp = [tiny.nv - 1, tiny.nv]
+    ncv = tiny.nv * tiny.nc
+    args = (tiny.psi_intp, tiny.mmtrx, tiny.e1, tiny.nv, tiny.nc, qp, 31, 3.0, 0.05, ncv, poly, 3)
+    sig = O.sigma_lanczos(*args, bands=bands)
+    ref = _sigma_sum_over_states(tiny, qp, 31, 3.0, 0.05, poly, bands=bands)
+    assert rel_err(sig, ref) < 1e-7
+    rest = O.sigma_lanczos(*args, bands=(0, bands[0])) + O.sigma_lanczos(*args, bands=(bands[1], tiny.nb))
+    assert rel_err(sig + rest, O.sigma_lanczos(*args)) < 1e-12
 
 
 def test_pvectors_match_definition(O, tiny):
@@ -174,6 +191,26 @@ def test_spline_cubic_interior(O):
         f = np.sin(2 * x) + 1j * np.cos(x)
         fx, _ = O.spline_evaluate(x, f, 0.123)
         assert abs(fx - (np.sin(0.246) + 1j * np.cos(0.123))) < tol
+
+
+def test_spline_exact_on_cubic_table_interior(O):
+    """spline.f90 takes its boundary slopes from the end pairs, so a cubic
+    table is not reproduced near the ends; the boundary error decays by about
+    2 - sqrt(3) per interval, and in the interior of a 200-point table the
+    spline equals the cubic to rounding (a wrong second-derivative system
+    fails this).  The returned slope is the central difference of Re f over
+    +-0.01 (printsigma's derivative step), which for the cubic term 0.3 E^3
+    exceeds the derivative by exactly 0.3 * 0.01^2."""
+    x = np.linspace(-3.0, 3.0, 200)
+
+    def cub(t):
+        return (0.3 * t ** 3 - 1.1 * t ** 2 + 0.5 * t - 2.0) + 1j * (-0.2 * t ** 3 + 0.7 * t)
+
+    f = cub(x)
+    for xx in (-1.0, 0.0, 0.37, 1.234, 2.0):
+        fx, df = O.spline_evaluate(x, f, xx)
+        assert abs(fx - cub(xx)) < 1e-12
+        assert abs(df - (0.9 * xx ** 2 - 2.2 * xx + 0.5 + 0.3e-4)) < 1e-9
 
 
 def test_qp_solve_matches_direct_root(O):
@@ -233,6 +270,38 @@ def test_isdf_kernel_symmetric_psd(O, tiny):
     M = O.isdf_kernel(tiny.zeta, tiny.gidx, tiny.nfft, tiny.h, tiny.trunc)
     assert rel_err(M, M.T) < 1e-10
     assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > -1e-8 * np.abs(M).max()
+
+
+@pytest.mark.parametrize("h", [0.25, 0.3])
+@pytest.mark.parametrize("ntrans", [1, 2])
+def test_isdf_kernel_scale_gaussian_pair_energies(O, h, ntrans):
+    """Absolute scale of M = (ntrans/h^3) zeta^T V zeta
+    (isdf_parallel_sym_UltraLowMem.F90:312, 865-867) with Gaussian columns
+    zeta_mu(r) = exp(-a |r - R_mu|^2).  Sums over grid points are integrals /
+    h^3, so M_mu,nu = (ntrans / h^6) E_mu,nu with the Coulomb energy of two
+    Gaussian charges Q = (pi/a)^(3/2) (e^2 = 2, Ry):
+    E = 2 Q^2 erf(sqrt(a/2) d) / d, and 2 Q^2 2 sqrt(a/2) / sqrt(pi) at d = 0.
+    A dropped or doubled h^3, a missing ntrans or a wrong e^2 fails by a
+    factor of at least 2; at two spacings the h dependence is fixed too."""
+    from math import erf, pi, sqrt
+
+    from nanogw_amd import synth
+
+    a = 2.0
+    g = synth.sphere_grid(4.5, h)
+    _, nfft, trunc = synth.fft_box(g, h)
+    r = g * h
+    cen = np.array([[0.0, 0.0, 0.0], [1.2, 0.0, 0.0], [0.0, -0.9, 0.7], [-1.0, 0.5, -0.6]])
+    zeta = np.asfortranarray(np.stack([np.exp(-a * ((r - c) ** 2).sum(1)) for c in cen], axis=1))
+    M = O.isdf_kernel(zeta, g, nfft, h, trunc, ntrans=ntrans)
+    q2 = (pi / a) ** 3
+    ref = np.empty((4, 4))
+    for i in range(4):
+        for j in range(4):
+            d = float(np.linalg.norm(cen[i] - cen[j]))
+            ref[i, j] = 2.0 * q2 * (erf(sqrt(a / 2) * d) / d if d > 0 else 2.0 * sqrt(a / 2) / sqrt(pi))
+    ref *= ntrans / h ** 6
+    assert np.abs(M / ref - 1.0).max() < 1e-8
 
 
 def test_cvt_lattice_lloyd_step_by_brute_force(O):
