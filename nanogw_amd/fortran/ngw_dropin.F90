@@ -51,6 +51,10 @@ module ngw_session
   type(c_ptr), save :: ngw_ctx = c_null_ptr
   type(c_ptr), save :: up_psi = c_null_ptr, up_m = c_null_ptr
   integer, save :: up_nintp = -1, up_nband = -1, up_nval = -1
+  ! values the resident factored operator was built from: e1(1:nband) and
+  ! samples of Psi_intp and Mmtrx (in-place updates between calls, e.g. the
+  ! energies an SCGW iteration rewrites, trigger a re-upload)
+  real(c_double), allocatable, save :: up_e1(:), up_psamp(:), up_pmsamp(:)
   integer, save :: ngw_device = 0
   integer, save :: ngw_precision = 0
   ! explicit-operator state of the single-vector routines
@@ -139,14 +143,36 @@ contains
     end do
   end subroutine ngw_pairs
 
-  ! Upload the operator when it is not already resident.
-  subroutine ngw_ensure(n_intp, ld_psi, psi, ld_m, mmtrx, nband, nval, e1)
-    integer, intent(in) :: n_intp, ld_psi, ld_m, nband, nval
+  ! Upload the factored operator unless the same one is resident: same
+  ! arrays and sizes, and the same e1(1:nband) and samples of Psi_intp and
+  ! Mmtrx as at the last upload.  The library forms sqrtR from e1 as the
+  ! reference drivers do (calculate_sigma_lanczos_BLOCK.F90z:199-205); a
+  ! caller's sqrtR that disagrees with wfn%e1 stops with a message.
+  subroutine ngw_ensure(n_intp, ld_psi, psi, ld_m, mmtrx, nband, nval, e1, ncv, sqrtr)
+    integer, intent(in) :: n_intp, ld_psi, ld_m, nband, nval, ncv
     real(c_double), intent(in), target :: psi(ld_psi, *), mmtrx(ld_m, *)
-    real(c_double), intent(in) :: e1(*)
+    real(c_double), intent(in) :: e1(*), sqrtr(ncv)
+    real(c_double), allocatable :: ps(:), ms(:)
+    integer :: icv, iv, ic
+    real(c_double) :: ref
     call ngw_start()
+    do icv = 1, ncv
+      iv = mod(icv - 1, nval) + 1
+      ic = nval + (icv - 1) / nval + 1
+      ref = sqrt(e1(ic) - e1(iv))
+      if (abs(sqrtr(icv) - ref) > 1.0d-12 * max(1.0d0, ref)) then
+        write (6, *) 'libngw: sqrtR does not match wfn%e1 at pair', icv, sqrtr(icv), ref
+        stop 1
+      end if
+    end do
+    call ngw_sample(psi, ld_psi * nband, ps)
+    call ngw_sample(mmtrx, ld_m * n_intp, ms)
     if (c_associated(up_psi, c_loc(psi(1, 1))) .and. c_associated(up_m, c_loc(mmtrx(1, 1))) .and. &
-        up_nintp == n_intp .and. up_nband == nband .and. up_nval == nval) return
+        up_nintp == n_intp .and. up_nband == nband .and. up_nval == nval) then
+      if (size(up_psamp) == size(ps) .and. size(up_pmsamp) == size(ms)) then
+        if (all(up_e1 == e1(1:nband)) .and. all(up_psamp == ps) .and. all(up_pmsamp == ms)) return
+      end if
+    end if
     call ngw_check(ngw_set_isdf(ngw_ctx, int(n_intp, c_int), int(nband, c_int), int(nval, c_int), &
                                 psi, int(ld_psi, c_int), mmtrx, int(ld_m, c_int), e1), 'ngw_set_isdf')
     call ngw_check(ngw_set_precision(ngw_ctx, int(ngw_precision, c_int)), 'ngw_set_precision')
@@ -155,6 +181,9 @@ contains
     up_nintp = n_intp
     up_nband = nband
     up_nval = nval
+    up_e1 = e1(1:nband)
+    call move_alloc(ps, up_psamp)
+    call move_alloc(ms, up_pmsamp)
     up_c = c_null_ptr
     up_cnintp = -1
     up_lncv = -1
@@ -283,7 +312,8 @@ subroutine matvec_isdf_BLOCK(vec, Hvec, ncv_loc, isdf_in, wfn, cvpair, ncv, sqrt
   if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
   call ngw_pairs(cvpair, 4, ncv, nval, nband)
   call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
-                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1, &
+                  ncv_loc, sqrtR)
   call ngw_check(ngw_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), int(blksz, c_int), 0_c_int), &
                  'matvec_isdf_BLOCK')
 end subroutine matvec_isdf_BLOCK
@@ -306,7 +336,8 @@ subroutine polynomial_matvec_isdf_BLOCK(vec, Hvec, ncv_loc, polynomial, npoly, i
   if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
   call ngw_pairs(cvpair, 4, ncv, nval, nband)
   call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
-                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1, &
+                  ncv_loc, sqrtR)
   call ngw_check(ngw_polynomial_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), polynomial, &
                                                   int(npoly, c_int), int(blksz, c_int), 0_c_int), &
                  'polynomial_matvec_isdf_BLOCK')
@@ -332,7 +363,8 @@ subroutine lanczos_spectra_isdf_BLOCK(v0, v0_norm, ncv_loc, zz, nz, niter, isdf_
   if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
   call ngw_pairs(cvpair, 4, ncv, nval, nband)
   call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
-                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1, &
+                  ncv_loc, sqrtR)
   call ngw_check(ngw_lanczos_spectra_isdf_block(ngw_ctx, v0, v0_norm, int(ncv_loc, c_int), zz, int(nz, c_int), &
                                                 int(niter, c_int), polynomial, int(npoly, c_int), spectra, &
                                                 int(blksz, c_int), 0_c_int, c_null_ptr, c_null_ptr), &
@@ -412,7 +444,8 @@ subroutine matvec_isdf_UltraLowMem(vec, Hvec, ncv_loc, isdf_in, wfn, cvpair, sqr
   if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
   call ngw_pairs(cvpair, 2, ncv_loc, nval, nband)
   call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
-                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1, &
+                  ncv_loc, sqrtR)
   call ngw_check(ngw_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), 1_c_int, 0_c_int), &
                  'matvec_isdf_UltraLowMem')
 end subroutine matvec_isdf_UltraLowMem
@@ -435,7 +468,8 @@ subroutine polynomial_matvec_isdf_UltraLowMem(vec, Hvec, ncv_loc, polynomial, np
   if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
   call ngw_pairs(cvpair, 2, ncv_loc, nval, nband)
   call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
-                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1, &
+                  ncv_loc, sqrtR)
   call ngw_check(ngw_polynomial_matvec_isdf_block(ngw_ctx, vec, Hvec, int(ncv_loc, c_int), polynomial, &
                                                   int(npoly, c_int), 1_c_int, 0_c_int), &
                  'polynomial_matvec_isdf_UltraLowMem')
@@ -462,7 +496,8 @@ subroutine lanczos_spectra_isdf_UltraLowMem(v0, v0_norm, ncv_loc, zz, nz, niter,
   if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
   call ngw_pairs(cvpair, 2, ncv_loc, nval, nband)
   call ngw_ensure(isdf_in%n_intp_r, size(isdf_in%Psi_intp_loc, 1), isdf_in%Psi_intp_loc(:, :, 1, 1), &
-                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1)
+                  size(isdf_in%Mmtrx_loc, 1), isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, 1), nband, nval, wfn%e1, &
+                  ncv_loc, sqrtR)
   call ngw_lanczos_single(v0, v0_norm, ncv_loc, zz, nz, niter, polynomial, npoly, spectra, &
                           'lanczos_spectra_isdf_UltraLowMem')
 end subroutine lanczos_spectra_isdf_UltraLowMem

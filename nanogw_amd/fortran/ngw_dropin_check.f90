@@ -6,7 +6,10 @@
 ! as Zcalculate_sigma_lanczos does (tmp_Cmtrx built from Psi, here over the
 ! pair list in reverse order), then their UltraLowMem variants, then the
 ! lowcomsym routines on a two-irrep blocking, and writes the results to a
-! stream file (argument 2).
+! stream file (argument 2).  Between the BLOCK calls and the single-vector
+! ones the conduction energies in wfn%e1 are raised by 0.1 Ry in place for
+! one more matvec_isdf_BLOCK call (the update an SCGW iteration makes), then
+! restored: the drop-in must rebuild its resident operator both times.
 program ngw_dropin_check
   use ngw_dropin_selftest_types
   use ngw_session, only: ngw_release
@@ -27,6 +30,7 @@ program ngw_dropin_check
   real(dp), allocatable :: hu(:), pu(:), veig2(:)
   complex(dp), allocatable :: specu(:), speigu(:)
   character(len=512) :: fin, fout
+  real(dp), allocatable :: hvec_e(:, :), sqrtr_e(:), e1_save(:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -55,6 +59,16 @@ program ngw_dropin_check
   v0norm = sqrt(sum(vec**2, dim=1))
   call lanczos_spectra_isdf_BLOCK(vec, v0norm, ncv, zz, nz, niter, isdf_in, wfn, cvpair, ncv, sqrtR, poly, &
                                   npoly, spectra, dummy, dummy, dummy, dummy, tmp_cvpair, blksz)
+  ! energies changed in place between calls
+  allocate (hvec_e(ncv, blksz), sqrtr_e(ncv), e1_save(nb))
+  e1_save = wfn%e1
+  wfn%e1(nv + 1:nb) = wfn%e1(nv + 1:nb) + 0.1_dp
+  do icv = 1, ncv
+    sqrtr_e(icv) = sqrt(wfn%e1(cvpair(2, icv)) - wfn%e1(cvpair(1, icv)))
+  end do
+  call matvec_isdf_BLOCK(vec, hvec_e, ncv, isdf_in, wfn, cvpair, ncv, sqrtr_e, dummy, dummy, dummy, dummy, &
+                         tmp_cvpair, blksz)
+  wfn%e1 = e1_save
   ! single-vector routines: pair jcv = ncv + 1 - icv
   do icv = 1, ncv
     jcv = ncv + 1 - icv
@@ -128,6 +142,6 @@ program ngw_dropin_check
                                       2, 1, gvec)
   call ngw_release()
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u) hvec, spectra, h1, p1, spec1, speig, hu, pu, specu, speigu, hs, hst, ps, specs, spect
+  write (u) hvec, spectra, h1, p1, spec1, speig, hu, pu, specu, speigu, hs, hst, ps, specs, spect, hvec_e
   close (u)
 end program ngw_dropin_check

@@ -67,7 +67,8 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     pairs = [(v, cl[c]) for v in range(nv1) for c in range(nc1)]
     pairs += [(v, cl[c]) for v in range(nv1, nv) for c in range(nc1, nc)]
     nsub = len(pairs)
-    blk = raw[o4:].reshape(-1)
+    hvec_e = raw[-ncv * blksz:].reshape(blksz, ncv).T
+    blk = raw[o4:-ncv * blksz].reshape(-1)
     hs = blk[:2 * nsub].reshape(2, nsub).T
     hst = blk[2 * nsub:4 * nsub].reshape(2, nsub).T
     ps = blk[4 * nsub:6 * nsub].reshape(2, nsub).T
@@ -77,6 +78,12 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     ref_s, _, _ = O.lanczos_spectra_block(m.psi_intp, m.mmtrx, s, vec, np.linalg.norm(vec, axis=0), zz, niter,
                                           poly, nv, nc)
     assert rel_err(hvec, ref_h) < 1e-11
+    # wfn%e1 changed in place between calls: the drop-in rebuilt its operator
+    e1e = m.e1.copy()
+    e1e[nv:] += 0.1
+    ref_e = O.matvec_block(m.psi_intp, m.mmtrx, O.sqrt_r(e1e, nv, nc), vec, nv, nc)
+    assert rel_err(hvec_e, ref_e) < 1e-11
+    assert rel_err(hvec_e, ref_h) > 1e-3
     assert rel_err(spec, ref_s) < 1e-6
     # single-vector routines on tmp_Cmtrx over the reversed pair list
     cm = np.asfortranarray(O.pair_cmtrx(m.psi_intp, nv, nc)[:, ::-1])
