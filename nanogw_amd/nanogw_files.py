@@ -17,6 +17,10 @@ read with the rules of NanoGW's ESDF parser (shared/esdf_module.f90):
 Keys and defaults follow SIGMA/input_s.F90:74-184 and shared/input_g.f90:
 66-94.
 
+V_xc: the diagonal (and off-diagonal) matrix elements of sigma_mtxel.dat
+that Zvxc reads when ``read_vxc_matrix_elements`` is set (SIGMA/vxc.F90z:
+56-111), list-directed, eV in the file, Ry returned.
+
 Output: the self-energy table of printsigma (SIGMA/printsigma.F90:167-188,
 file ``sigma_nostatic_<ik>_<it>``, ``_up_`` / ``_down_`` for spin) and its
 ``hmat_qp_nostatic`` line per state, with the same Fortran edit
@@ -25,6 +29,8 @@ descriptors.
 from __future__ import annotations
 
 import dataclasses
+
+import numpy as np
 
 RYD = 13.60569193  # shared/myconstants.F90:41
 
@@ -170,6 +176,7 @@ class SigmaInput:
     diag: list = dataclasses.field(default_factory=list)  # 1-based orbitals
     read_vxc: bool = False
     exchange_correlation: str = "gw"
+    scissors: list = dataclasses.field(default_factory=list)  # lines of the 'scissors' block
 
 
 def read_nanogw_in(path_or_text, is_text=False) -> SigmaInput:
@@ -197,7 +204,74 @@ def read_nanogw_in(path_or_text, is_text=False) -> SigmaInput:
     s.diag = parse_index_block(blk) if blk is not None else []
     s.read_vxc = f.defined("read_vxc_matrix_elements")
     s.exchange_correlation = esdf_reduce(f.string("exchange_correlation", "gw"))
+    sc = f.block("scissors")
+    s.scissors = list(sc) if sc is not None else []
     return s
+
+
+def _list_directed_records(text):
+    """Token lists of the records of a list-directed input file: values are
+    separated by blanks or commas, 'd' exponents read as 'e', and r*c repeats
+    expand to r copies of c (Fortran list-directed input)."""
+    recs = []
+    for line in text.splitlines():
+        out = []
+        for tok in line.replace(",", " ").split():
+            if tok.startswith("/"):
+                break
+            if "*" in tok:
+                r, c = tok.split("*", 1)
+                out.extend([c] * int(r))
+            else:
+                out.append(tok)
+        recs.append(out)
+    return recs
+
+
+def read_vxc_matrix_elements(path_or_text, ik, isp, diag_bands, offd_pairs=(), nstate=None, cplx=False,
+                             is_text=False):
+    """V_xc matrix elements from sigma_mtxel.dat, as Zvxc reads them with
+    read_vxc_matrix_elements set (SIGMA/vxc.F90z:56-111): every statement
+    ``read (23, *) ikp, jsp, ii, jj, re [, im]`` takes the next 5 (6 for
+    complex wave functions) values, starting a new record and continuing
+    over records as list-directed input does; lines of other k-points or
+    spins are skipped; ii == jj is a diagonal element, stored when band ii
+    is one of diag_bands; other pairs are stored when (ii, jj) is one of
+    offd_pairs; a later line overwrites an earlier one.  Values are eV in
+    the file and returned in Ry (/ ryd).  Bands are 1-based.  An element
+    that stays undefined raises, as Zvxc dies ('undefined Vxc matrix
+    elements').  Returns (vxc_diag, vxc_offd) complex arrays in the order
+    of diag_bands and offd_pairs."""
+    text = path_or_text if is_text else open(path_or_text).read()
+    big = 9.0e9
+    diag = np.full(len(diag_bands), big + 0j)
+    offd = np.full(len(offd_pairs), big + 0j)
+    inv_diag = {int(b): i for i, b in enumerate(diag_bands)}
+    inv_offd = {(int(a), int(b)): i for i, (a, b) in enumerate(offd_pairs)}
+    nval = 6 if cplx else 5
+    recs = _list_directed_records(text)
+    ri = 0
+    while ri < len(recs):
+        vals = []
+        while len(vals) < nval and ri < len(recs):
+            vals.extend(recs[ri])
+            ri += 1
+        if len(vals) < nval:
+            break  # end of file inside a statement ends the loop (end=10)
+        ikp, jsp, ii, jj = (int(v) for v in vals[:4])
+        re = float(vals[4].lower().replace("d", "e"))
+        im = float(vals[5].lower().replace("d", "e")) if cplx else 0.0
+        if ikp != ik or jsp != isp:
+            continue
+        if ii == jj:
+            if ii in inv_diag:
+                diag[inv_diag[ii]] = complex(re, im) / RYD
+        elif (ii, jj) in inv_offd:
+            offd[inv_offd[(ii, jj)]] = complex(re, im) / RYD
+    undefined = int(np.sum(diag.real == big) + np.sum(offd.real == big))
+    if undefined:
+        raise ValueError(f"ERROR! {undefined} undefined Vxc matrix elements for spin channel {isp}")
+    return diag, offd
 
 
 def _fortran_f(x: float, w: int, d: int) -> str:

@@ -188,6 +188,31 @@ def test_qp_newton_vs_oracle(ngw_ctx):
         assert res["ierr"][i] == r["ierr"]
 
 
+def test_qp_newton_vxc_from_sigma_mtxel(ngw_ctx, tmp_path):
+    """V_xc read from sigma_mtxel.dat (SIGMA/vxc.F90z:56-111) enters the QP
+    Newton solve on the device as printsigma uses it (f = E_dft + Sigma_x +
+    Sigma_c(E) - V_xc - E); the oracle solves with the same values."""
+    from nanogw_amd import nanogw_files as F
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(8)
+    nen, bands = 200, [4, 5, 6]
+    x = np.linspace(-1, 1, nen)
+    sig = np.stack([(rng.normal() + 0.4 * x + 0.2 * np.sin(2 * x)) + 0.05j * np.cos(x) for _ in bands], axis=1)
+    path = tmp_path / "sigma_mtxel.dat"
+    path.write_text("".join(f"1 1 {b} {b} {-11.0 + 0.7 * k:.6f}\n" for k, b in enumerate(bands)))
+    vxc_ry, _ = F.read_vxc_matrix_elements(str(path), 1, 1, bands)
+    vxc_ev = vxc_ry.real * F.RYD
+    edft = rng.uniform(-10, 0, len(bands))
+    sigx = rng.uniform(-5, 0, len(bands))
+    res = ngw_ctx.qp_newton(sig * 13.6, 27.2, edft, edft, sigx, vxc_ev)
+    res0 = ngw_ctx.qp_newton(sig * 13.6, 27.2, edft, edft, sigx, np.zeros(len(bands)))
+    for i in range(len(bands)):
+        r = O.qp_solve(sig[:, i] * 13.6, 27.2, edft[i], edft[i], sigx[i], vxc_ev[i])
+        assert abs(res["eqp"][i] - r["eqp"]) < 1e-9 and res["ierr"][i] == r["ierr"]
+        assert abs(res["eqp"][i] - res0["eqp"][i]) > 1.0  # V_xc of about -10 eV moves E_qp
+
+
 def test_errors_raise(ngw_ctx, mini):
     """Size mismatches and bad indices come back as statuses, not faults."""
     import ctypes as C

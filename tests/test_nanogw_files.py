@@ -137,3 +137,34 @@ def test_formats_match_a_fortran_runtime(tmp_path):
         assert F.sigma_row(n, occ, *vals) == line, (F.sigma_row(n, occ, *vals), line)
     for line, (n, eqp, isp, ik, edft, e0) in zip(lines[len(rows):], hm):
         assert F.hmat_row(n, eqp, isp, ik, edft, e0) == line, (F.hmat_row(n, eqp, isp, ik, edft, e0), line)
+
+
+def test_read_vxc_matrix_elements_list_directed():
+    """sigma_mtxel.dat as Zvxc reads it (SIGMA/vxc.F90z:56-111): 5 values
+    per read statement (6 for complex wave functions), continuing over
+    records, the rest of the last record skipped; other k-points and spins
+    ignored; later lines overwrite; eV -> Ry; undefined elements raise."""
+    text = "\n".join([
+        "1 1 3 3 -10.5",
+        "1 1 4 4 -9.25d0 ignored trailing values",
+        "2 1 3 3 -99.0",          # another k-point
+        "1 2 3 3 -98.0",          # another spin
+        "1, 1, 5, 5,",            # one statement over two records
+        "  -8.0",
+        "",
+        "1 1 3 4 0.75",           # off-diagonal
+        "1 1 4 4 -9.5",           # overwrites band 4
+        "1 1 2*6 -7.0",           # repeat count: ii = jj = 6
+    ])
+    d, o = F.read_vxc_matrix_elements(text, 1, 1, [3, 4, 5, 6], [(3, 4)], is_text=True)
+    assert np.allclose(d.real * F.RYD, [-10.5, -9.5, -8.0, -7.0]) and np.all(d.imag == 0)
+    assert np.allclose(o.real * F.RYD, [0.75])
+    with pytest.raises(ValueError, match="1 undefined"):
+        F.read_vxc_matrix_elements(text, 1, 1, [3, 7], is_text=True)
+    dc, _ = F.read_vxc_matrix_elements("1 1 2 2 -4.0 0.5\n", 1, 1, [2], cplx=True, is_text=True)
+    assert np.allclose(dc * F.RYD, [-4.0 + 0.5j])
+
+
+def test_scissors_block_of_nanogw_in():
+    s = F.read_nanogw_in("begin scissors\n1 1 4 0.5 -10.0 0.1\nend scissors\n", is_text=True)
+    assert s.scissors == ["1 1 4 0.5 -10.0 0.1"]
