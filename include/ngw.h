@@ -78,6 +78,19 @@ int ngw_set_isdf(ngw_context* ctx, int n_intp, int nband, int nval, const double
  * which does not amplify rounding the way monomial Horner does. */
 int ngw_set_precision(ngw_context* ctx, int precision);
 
+#define NGW_REORTH_NONE 0 /* the reference's plain three-term recurrence
+                             (shared/lanczos_spectra_BLOCK.F90:79-108) */
+#define NGW_REORTH_FULL 1 /* full reorthogonalisation: after w -= alpha q_k, two
+                             classical Gram-Schmidt passes of w against q_1..q_k */
+
+/* Reorthogonalisation of the Lanczos recurrences of ngw_lanczos_spectra_isdf_block
+ * and ngw_sigma_lanczos(_bands) (NGW_REORTH_*; default NONE, which is NanoGW's
+ * algorithm and the parity path).  No counterpart in NanoGW.  FULL keeps every
+ * Lanczos vector on the device, 8 * niter * ncv bytes per column; a run that
+ * does not fit returns NGW_ERR_NOMEM.  All arithmetic FP64 in every precision
+ * mode.  The batched engine (ngw_batch_*) runs the plain recurrence only. */
+int ngw_set_reorthogonalization(ngw_context* ctx, int mode);
+
 /*
  * Upload the operator in the explicit form of NanoGW's single-vector
  * routines, matvec_isdf (shared/apply_matvec.f90:25), polynomial_matvec_isdf

@@ -28,6 +28,7 @@ _SIGS = {
     "ngw_synchronize": ([_vp], C.c_int),
     "ngw_set_isdf": ([_vp, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp], C.c_int),
     "ngw_set_precision": ([_vp, C.c_int], C.c_int),
+    "ngw_set_reorthogonalization": ([_vp, C.c_int], C.c_int),
     "ngw_set_explicit_c": ([_vp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp], C.c_int),
     "ngw_matvec_isdf_block": ([_vp, _dp, _dp, C.c_int, C.c_int, C.c_int], C.c_int),
     "ngw_polynomial_matvec_isdf_block": ([_vp, _dp, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int],

@@ -328,6 +328,15 @@ int ngw_set_precision(ngw_context* ctx, int precision)
     return NGW_OK;
 }
 
+int ngw_set_reorthogonalization(ngw_context* ctx, int mode)
+{
+    if (!ctx) return fail(NGW_ERR_ARG, "null context");
+    if (mode != NGW_REORTH_NONE && mode != NGW_REORTH_FULL) return fail(NGW_ERR_ARG, "bad reorthogonalisation mode");
+    ctx->reorth = mode;
+    if (mode == NGW_REORTH_NONE) ctx->qbasis.release();
+    return NGW_OK;
+}
+
 int ngw_matvec_isdf_block_dev(ngw_context* ctx, const double* d_vec, double* d_hvec, long ld,
                               int blksz, int tamm_d)
 {

@@ -58,6 +58,86 @@ __global__ __launch_bounds__(RT) void lanczos_ab_kernel(double* q, double* w, lo
     }
 }
 
+// ---------------------------------------------------------------- full reorthogonalisation
+// Optional (off in the reference path): the step above split in two around
+// two classical Gram-Schmidt passes of w against every stored Lanczos
+// vector q_1..q_k of its column.  The basis of column c is Q + (c niter + j)
+// ld, j = 0..niter-1.  All sums are in a fixed order, so results do not
+// depend on the launch.
+//
+// Part 1: alpha = w.q, w -= alpha q, q_k stored as basis vector k - 1.
+__global__ __launch_bounds__(RT) void lanczos_a_kernel(const double* q, double* w, long ld, long n, double* a,
+                                                       double* b, int niter, int k, double* basis)
+{
+    __shared__ double sh[4];
+    const long col = blockIdx.x;
+    const double* qc = q + col * ld;
+    double* wc = w + col * ld;
+    double* bc = basis + ((long)col * niter + (k - 1)) * ld;
+    double s = 0.0;
+    for (long i = threadIdx.x; i < n; i += RT) s += wc[i] * qc[i];
+    const double al = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        a[(k - 1) + col * niter] = al;
+        if (k == 1) b[col * niter] = 1.0;
+    }
+    for (long i = threadIdx.x; i < n; i += RT) {
+        const double qi = qc[i];
+        wc[i] = wc[i] + (-al) * qi;
+        bc[i] = qi;
+    }
+}
+
+// h(j, col) = q_j . w for j < k: one workgroup per (j, col).
+__global__ __launch_bounds__(RT) void reorth_dot_kernel(const double* basis, const double* w, long ld, long n,
+                                                        int niter, int k, double* h)
+{
+    __shared__ double sh[4];
+    const int j = blockIdx.x;
+    const long col = blockIdx.y;
+    const double* qj = basis + ((long)col * niter + j) * ld;
+    const double* wc = w + col * ld;
+    double s = 0.0;
+    for (long i = threadIdx.x; i < n; i += RT) s += qj[i] * wc[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) h[j + col * niter] = t;
+}
+
+// w(i, col) -= sum_{j<k} q_j(i) h(j, col), j ascending.
+__global__ __launch_bounds__(RT) void reorth_update_kernel(const double* basis, double* w, long ld, long n,
+                                                           int niter, int k, const double* h)
+{
+    const long col = blockIdx.y;
+    const double* hc = h + (long)col * niter;
+    const double* bc = basis + (long)col * niter * ld;
+    double* wc = w + col * ld;
+    for (long i = blockIdx.x * (long)RT + threadIdx.x; i < n; i += (long)gridDim.x * RT) {
+        double s = 0.0;
+        for (int j = 0; j < k; ++j) s += bc[(long)j * ld + i] * hc[j];
+        wc[i] -= s;
+    }
+}
+
+// Part 2: beta = |w|, then the rotation into the next step (q = w / beta,
+// w = -beta q_k), as in lanczos_ab_kernel.
+__global__ __launch_bounds__(RT) void lanczos_b_kernel(double* q, double* w, long ld, long n, double* b, int niter,
+                                                       int k)
+{
+    __shared__ double sh[4];
+    const long col = blockIdx.x;
+    double* qc = q + col * ld;
+    double* wc = w + col * ld;
+    double s2 = 0.0;
+    for (long i = threadIdx.x; i < n; i += RT) s2 += wc[i] * wc[i];
+    const double be = sqrt(block_sum(s2, sh));
+    if (threadIdx.x == 0) b[k + col * niter] = be;
+    for (long i = threadIdx.x; i < n; i += RT) {
+        const double wi = wc[i], qi = qc[i];
+        qc[i] = wi / be;
+        wc[i] = -qi * be;
+    }
+}
+
 __global__ __launch_bounds__(RT) void column_norm_kernel(const double* v, long ld, long n,
                                                          double* norm2, double* norm)
 {
@@ -237,10 +317,36 @@ int lanczos_run(ngw_context* ctx, double* d_q, double* d_w, double* d_h0, double
 {
     hipStream_t s = ctx->stream;
     const long n = ctx->ncv;
+    double* basis = nullptr;
+    double* h = nullptr;
+    if (ctx->reorth) {
+        const double need = 8.0 * ((double)niter * ld * nvec + (double)niter * nvec);
+        if (ctx->qbasis.ensure((size_t)need) != NGW_OK)
+            return fail(NGW_ERR_NOMEM, "reorthogonalisation keeps every Lanczos vector: " +
+                                           std::to_string(need / 1e9) + " GB for " + std::to_string(nvec) +
+                                           " columns x " + std::to_string(niter) + " steps does not fit");
+        basis = ctx->qbasis.as<double>();
+        h = basis + (size_t)niter * ld * nvec;
+    }
     for (int k = 1; k <= niter; ++k) {
         NGW_TRY(poly_apply(ctx, d_q, d_w, d_h0, d_h1, d_h2, ld, nvec, poly, npoly, tda, k > 1));
         ProfScope ps(ctx, KC_LANCZOS);
-        hipLaunchKernelGGL(lanczos_ab_kernel, dim3(nvec), dim3(RT), 0, s, d_q, d_w, ld, n, d_a, d_b, niter, k);
+        if (!basis) {
+            hipLaunchKernelGGL(lanczos_ab_kernel, dim3(nvec), dim3(RT), 0, s, d_q, d_w, ld, n, d_a, d_b, niter, k);
+        } else {
+            hipLaunchKernelGGL(lanczos_a_kernel, dim3(nvec), dim3(RT), 0, s, d_q, d_w, ld, n, d_a, d_b, niter, k,
+                               basis);
+            if (k < niter) {
+                const int gx = (int)std::min<long>((n + RT - 1) / RT, 64);
+                for (int pass = 0; pass < 2; ++pass) {
+                    hipLaunchKernelGGL(reorth_dot_kernel, dim3(k, nvec), dim3(RT), 0, s, basis, d_w, ld, n, niter,
+                                       k, h);
+                    hipLaunchKernelGGL(reorth_update_kernel, dim3(gx, nvec), dim3(RT), 0, s, basis, d_w, ld, n,
+                                       niter, k, h);
+                }
+                hipLaunchKernelGGL(lanczos_b_kernel, dim3(nvec), dim3(RT), 0, s, d_q, d_w, ld, n, d_b, niter, k);
+            }
+        }
         NGW_HIP(hipGetLastError());
     }
     return NGW_OK;

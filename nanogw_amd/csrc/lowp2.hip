@@ -670,6 +670,29 @@ __global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, c
     }
 }
 
+// 16-byte accesses of four consecutive elements (FP32 or FP64), widened to
+// or narrowed from FP64.
+__device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+__device__ __forceinline__ void ld4(const float* p, double (&o)[4])
+{
+    const f32x4 v = *reinterpret_cast<const f32x4*>(p);
+    o[0] = (double)v.x; o[1] = (double)v.y; o[2] = (double)v.z; o[3] = (double)v.w;
+}
+__device__ __forceinline__ void ld4(const double* p, double (&o)[4])
+{
+    const double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4(float* p, const double (&r)[4])
+{
+    *reinterpret_cast<f32x4*>(p) = f32x4{(float)r[0], (float)r[1], (float)r[2], (float)r[3]};
+}
+__device__ __forceinline__ void st4(double* p, const double (&r)[4])
+{
+    reinterpret_cast<double2*>(p)[0] = double2{r[0], r[1]};
+    reinterpret_cast<double2*>(p)[1] = double2{r[2], r[3]};
+}
+
 // 128 x 128 tiles keep three workgroups per CU: the register budget is 168
 // VGPRs (the swizzled-LDS variant otherwise lands at 170, two waves).
 template <int KIND, int BM, int BN, int D, int PM, int FW, int SW>
@@ -714,21 +737,104 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
         const TX* xp = reinterpret_cast<const TX*>(sg.x);
         const TY* yp = reinterpret_cast<const TY*>(sg.y);
         TO* op_ = reinterpret_cast<TO*>(sg.out);
-        for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
-            const int c = m0 + row, j = n0 + col;
-            if (c < op.nc && j < N) {
-                const int b = j / nv, v = j - b * nv;
-                const long idx = v + (long)c * nv, i = idx + (long)b * op.ld;
-                double Y = (double)val * (double)sigc[c];
-                if constexpr (KIND == LP_F16) Y *= (double)op.wsc[j] * F16_UNSCALE;
-                const double sv = ss ? ss[idx] : 1.0;
-                const double xv = (double)xp[i];
-                const double hx = dd[idx] * (ss ? sv * xv : xv) + (beta * sv) * Y;
-                double r = sg.alpha * hx + sg.bx * xv;
-                if (yp) r = r + sg.by * (double)yp[i];
-                if (use_q) r = r + sg.gamma * op.q[i];
-                if (acc_on) r = (double)op_[i] + r;
-                op_[i] = (TO)r;
+        // four consecutive pairs icv = v..v+3 of one c share a b when nv is a
+        // multiple of four (tiles start at multiples of four): then every
+        // array is read and written with 16-byte accesses
+        const bool vec = (nv & 3) == 0 && (op.ld & 3) == 0 && aligned16(xp) && aligned16(yp) && aligned16(op_) &&
+                         aligned16(use_q ? op.q : nullptr) && aligned16(dd) && aligned16(ss) &&
+                         (KIND != LP_F16 || aligned16(op.wsc));
+        if (!vec) {
+            for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
+                const int c = m0 + row, j = n0 + col;
+                if (c < op.nc && j < N) {
+                    const int b = j / nv, v = j - b * nv;
+                    const long idx = v + (long)c * nv, i = idx + (long)b * op.ld;
+                    double Y = (double)val * (double)sigc[c];
+                    if constexpr (KIND == LP_F16) Y *= (double)op.wsc[j] * F16_UNSCALE;
+                    const double sv = ss ? ss[idx] : 1.0;
+                    const double xv = (double)xp[i];
+                    const double hx = dd[idx] * (ss ? sv * xv : xv) + (beta * sv) * Y;
+                    double r = sg.alpha * hx + sg.bx * xv;
+                    if (yp) r = r + sg.by * (double)yp[i];
+                    if (use_q) r = r + sg.gamma * op.q[i];
+                    if (acc_on) r = (double)op_[i] + r;
+                    op_[i] = (TO)r;
+                }
+            });
+            return;
+        }
+        // The tile passes through LDS one accumulator fragment row at a time:
+        // in pass q every wave writes its fragment row q (16 tile rows, so
+        // 32 rows in all: local rows 0-15 from the upper half of the tile,
+        // 16-31 from the lower), then every thread takes runs of four
+        // consecutive columns of one row (a wave covers whole rows, so each
+        // access of the wave is one contiguous run of x, y, q and out per
+        // pair block).  The fragment index is a compile-time constant, so a
+        // pass frees its accumulators.  Pitch = BN + 16 floats: the 16 lanes
+        // of an accumulator row and the next row's 16 fall on disjoint banks.
+        constexpr int FM = BM / 32, EPR = 32, PITCH = BN + 16, G = BN / 4, PER = EPR * G / GNT;
+        static_assert(EPR * G % GNT == 0, "epilogue row blocks");
+        static_assert(sizeof(sm) >= sizeof(float) * EPR * PITCH, "epilogue staging exceeds the slab LDS");
+        float(*stg)[PITCH] = reinterpret_cast<float(*)[PITCH]>(&sm);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int hrow = (wave & 1) * 16, wn = (wave >> 1) * (BN / 2);
+        static_for<FM>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (q > 0) __syncthreads();
+#pragma unroll
+            for (int jf = 0; jf < BN / 32; ++jf)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    stg[hrow + 4 * (lane >> 4) + r][wn + 16 * jf + (lane & 15)] = acc.v[q][jf][r];
+            __syncthreads();
+#pragma unroll 1
+            for (int k = 0; k < PER; ++k) {
+                const int e = threadIdx.x + GNT * k;
+                const int rl = e / G, cg = e - rl * G;
+                const int c = m0 + (rl < 16 ? 16 * q + rl : BM / 2 + 16 * q + rl - 16), j0 = n0 + 4 * cg;
+                if (c >= op.nc || j0 >= N) continue;
+                const f32x4 v4 = *reinterpret_cast<const f32x4*>(&stg[rl][4 * cg]);
+                const int b = j0 / nv, v0 = j0 - b * nv;
+                const long idx = v0 + (long)c * nv, i = idx + (long)b * op.ld;
+                // the stage's terms are added in the order of the element-wise
+                // form above, each array loaded just before it is needed
+                const double sc = (double)sigc[c];
+                const float va[4] = {v4.x, v4.y, v4.z, v4.w};
+                double res[4], xv[4], t4[4];
+                {
+                    float ws[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+                    if constexpr (KIND == LP_F16) {
+                        const f32x4 w4 = *reinterpret_cast<const f32x4*>(op.wsc + j0);
+                        ws[0] = w4.x; ws[1] = w4.y; ws[2] = w4.z; ws[3] = w4.w;
+                    }
+                    double sv[4] = {1.0, 1.0, 1.0, 1.0};
+                    ld4(xp + i, xv);
+                    ld4(dd + idx, t4);
+                    if (ss) ld4(ss + idx, sv);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        double Y = (double)va[u] * sc;
+                        if constexpr (KIND == LP_F16) Y *= (double)ws[u] * F16_UNSCALE;
+                        const double hx = t4[u] * (ss ? sv[u] * xv[u] : xv[u]) + (beta * sv[u]) * Y;
+                        res[u] = sg.alpha * hx + sg.bx * xv[u];
+                    }
+                }
+                if (yp) {
+                    ld4(yp + i, t4);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) res[u] = res[u] + sg.by * t4[u];
+                }
+                if (use_q) {
+                    ld4(op.q + i, t4);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) res[u] = res[u] + sg.gamma * t4[u];
+                }
+                if (acc_on) {
+                    ld4(op_ + i, t4);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) res[u] = t4[u] + res[u];
+                }
+                st4(op_ + i, res);
             }
         });
     };

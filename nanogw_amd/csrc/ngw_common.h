@@ -72,6 +72,10 @@ struct ngw_context {
     int n_intp = 0, nband = 0, nval = 0, ncond = 0;
     long ncv = 0;
     int precision = ngw::PREC_F64;
+    // Lanczos reorthogonalisation (NGW_REORTH_*): off in the reference path;
+    // full keeps every Lanczos vector in qbasis (niter x ncv x columns FP64)
+    int reorth = 0;
+    ngw::DevBuf qbasis;
     ngw::DevBuf psi;      // n_intp x nband, column-major, ld = ld_psi
     int ld_psi = 0;
     ngw::DevBuf mmat;     // n_intp x n_intp, ld = ld_m

@@ -45,6 +45,11 @@ module ngw_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: precision
     end function ngw_set_precision
+    integer(c_int) function ngw_set_reorthogonalization(ctx, mode) bind(C, name='ngw_set_reorthogonalization')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function ngw_set_reorthogonalization
 
     integer(c_int) function ngw_set_explicit_c(ctx, n_intp, ncv, cmtrx, ld_c, mmtrx, ld_m, sqrtr) &
       bind(C, name='ngw_set_explicit_c')

@@ -279,6 +279,19 @@ contains
     up_cnintp = -1
   end subroutine ngw_upload_lowcomsym
 
+  ! Largest explicit pair matrix the lowcomsym routines form (bytes): 2 GiB,
+  ! or NGW_LOWCOMSYM_MAX_GIB from the environment.
+  real(c_double) function ngw_lowcomsym_limit()
+    character(len=32) :: val
+    integer :: st, ios
+    real(c_double) :: gib
+    ngw_lowcomsym_limit = 2.0d0 * 1024.0d0**3
+    call get_environment_variable('NGW_LOWCOMSYM_MAX_GIB', val, status=st)
+    if (st /= 0) return
+    read (val, *, iostat=ios) gib
+    if (ios == 0 .and. gib > 0.0d0) ngw_lowcomsym_limit = gib * 1024.0d0**3
+  end function ngw_lowcomsym_limit
+
   subroutine ngw_release()
     integer(c_int) :: st
     if (c_associated(ngw_ctx)) st = ngw_destroy(ngw_ctx)
@@ -537,6 +550,12 @@ subroutine ngw_lowcomsym_operator(ncv_loc, isdf_in, sqrtR, nmrep, gvec)
     samp = [samp, isdf_in%Mmtrx_loc(k, 1 + mod((k - 1) * 7919, n), 1, 1, 1, 1, nmrep)]
   end do
   if (ngw_lowcomsym_resident(kv, kc, km, nmrep, n, ncv_loc, sqrtR, samp)) return
+  if (8.0d0 * real(n, 8) * real(ncv_loc, 8) > ngw_lowcomsym_limit()) then
+    write (6, '(a, f8.2, a, f8.2, a)') ' libngw drop-in: the lowcomsym pair matrix would take ', &
+      8.0d0 * real(n, 8) * real(ncv_loc, 8) / 1024.0d0**3, ' GiB on the host and the device, above the ', &
+      ngw_lowcomsym_limit() / 1024.0d0**3, ' GiB limit (environment NGW_LOWCOMSYM_MAX_GIB)'
+    stop 1
+  end if
   allocate (cm(n, ncv_loc))
   icv = 0
   do jvrep = 1, gvec%syms%ntrans

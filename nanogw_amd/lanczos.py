@@ -111,6 +111,12 @@ class IsdfContext:
         p = PREC[precision] if isinstance(precision, str) else int(precision)
         _lib.check(self._lib.ngw_set_precision(self._h, p), "ngw_set_precision")
 
+    def set_reorthogonalization(self, full: bool):
+        """Full reorthogonalisation of the Lanczos recurrences (no NanoGW
+        counterpart; off = the reference algorithm)."""
+        _lib.check(self._lib.ngw_set_reorthogonalization(self._h, 1 if full else 0),
+                   "ngw_set_reorthogonalization")
+
     @property
     def ncv(self):
         return self._ncv

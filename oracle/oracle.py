@@ -158,6 +158,45 @@ def lanczos_dense(h, v0, zz, niter):
     return spec, a, b
 
 
+def lanczos_reorth(apply, v0, zz, niter, reorth=True):
+    """Lanczos with full reorthogonalisation (numpy, one column at a time).
+
+    NanoGW has no reorthogonalisation; this is the textbook scheme (Parlett,
+    The Symmetric Eigenvalue Problem, ch. 13) in the recurrence order of
+    shared/lanczos_spectra_BLOCK.F90:79-108: w = A q_k - beta_k q_{k-1};
+    alpha_k = w.q_k; w -= alpha_k q_k; then, with reorth, two classical
+    Gram-Schmidt passes w -= Q (Q^T w) against Q = [q_1 .. q_k] ("twice is
+    enough", Giraud, Langou, Rozloznik 2005); beta_{k+1} = |w|,
+    q_{k+1} = w / beta_{k+1}.  b(1) = 1 as in the reference.  `apply`
+    maps an (n,) vector to A times it.  Returns spectra (the reference's
+    continued fraction of a, b at zz), a, b and Q (n, niter)."""
+    v0 = np.asarray(v0, np.float64)
+    n = v0.shape[0]
+    a = np.zeros(niter)
+    b = np.zeros(niter)
+    Q = np.zeros((n, niter))
+    q = v0 / np.linalg.norm(v0)
+    qold = np.zeros(n)
+    beta = 0.0
+    for k in range(niter):
+        w = apply(q) - beta * qold
+        al = float(w @ q)
+        a[k] = al
+        if k == 0:
+            b[0] = 1.0
+        Q[:, k] = q
+        if k + 1 == niter:
+            break
+        w = w - al * q
+        if reorth:
+            for _ in range(2):
+                w = w - Q[:, :k + 1] @ (Q[:, :k + 1].T @ w)
+        beta = float(np.linalg.norm(w))
+        b[k + 1] = beta
+        qold, q = q, w / beta
+    return continued_fraction(a, b, zz), a, b, Q
+
+
 def lanczos_spectra_block(psi, mmat, sqrtR, v0, v0_norm, zz, niter, poly, nv, nc, tda=False,
                           rebuild=False):
     """zz: (nz, blksz) complex.  Returns spectra (nz, blksz), a, b (niter, blksz)."""

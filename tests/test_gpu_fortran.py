@@ -15,6 +15,39 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 
+def _write_inputs(fin, m, blksz, niter, poly, vec, zz, veig):
+    """Input file of the check program: sizes, then column-major arrays."""
+    with open(fin, "wb") as f:
+        np.array([m.n_intp, m.nv, m.nc, blksz, zz.shape[0], niter, len(poly) - 1], np.int32).tofile(f)
+        for a in (m.psi_intp, m.mmtrx, m.e1, vec, poly):
+            np.asfortranarray(a, np.float64).T.tofile(f)  # column-major bytes
+        np.asfortranarray(zz, np.complex128).T.tofile(f)
+        veig.tofile(f)
+
+
+def test_fortran_dropin_lowcomsym_size_limit(gpu, mini, tmp_path):
+    """The lowcomsym routines form the irrep's pair matrix on the host; above
+    the stated limit (NGW_LOWCOMSYM_MAX_GIB, default 2 GiB) they stop with a
+    message before allocating it.  Here the limit is set below the check
+    program's tiny pair matrix."""
+    from nanogw_amd.build import DROPIN_CHECK
+    from nanogw_amd.lanczos import lanczos_polynomial
+
+    if not os.path.exists(DROPIN_CHECK):
+        pytest.skip("Fortran drop-in check program not built (no amdflang)")
+    m = mini
+    ncv, blksz, niter = m.nv * m.nc, 3, 20
+    poly = lanczos_polynomial(m.e1, m.nv, 12)
+    rng = np.random.default_rng(11)
+    zz = rng.uniform(-1, 3, size=(17, blksz)) + 0.03j
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_inputs(fin, m, blksz, niter, poly, rng.normal(size=(ncv, blksz)), zz, rng.normal(size=ncv))
+    env = dict(os.environ, NGW_LOWCOMSYM_MAX_GIB="1e-9")
+    r = subprocess.run([DROPIN_CHECK, str(fin), str(fout)], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode != 0
+    assert "lowcomsym pair matrix would take" in r.stdout + r.stderr
+
+
 def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     from nanogw_amd.build import DROPIN_CHECK
     from nanogw_amd.lanczos import lanczos_polynomial
@@ -37,12 +70,7 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     w, Z = np.linalg.eigh(0.5 * (hd + hd.T))
     veig = Z[::-1, 5].copy()
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    with open(fin, "wb") as f:
-        np.array([nmu, nv, nc, blksz, zz.shape[0], niter, len(poly) - 1], np.int32).tofile(f)
-        for a in (m.psi_intp, m.mmtrx, m.e1, vec, poly):
-            np.asfortranarray(a, np.float64).T.tofile(f)  # column-major bytes
-        np.asfortranarray(zz, np.complex128).T.tofile(f)
-        veig.tofile(f)
+    _write_inputs(fin, m, blksz, niter, poly, vec, zz, veig)
     r = subprocess.run([DROPIN_CHECK, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(fout, np.float64)

@@ -80,6 +80,56 @@ def test_lanczos_moments_and_exact_limit(O):
     assert rel_err(spec, exact) < 1e-9
 
 
+def _separated_extremes(n, seed):
+    """Symmetric matrix whose spectrum has three well-separated top eigenvalues
+    over a dense bulk: the case where the plain recurrence loses orthogonality
+    once the top Ritz values converge and then repeats them (ghosts)."""
+    rng = np.random.default_rng(seed)
+    lam = np.concatenate([np.linspace(0.0, 1.0, n - 3), [3.0, 5.0, 8.0]])
+    Z, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    h = (Z * lam) @ Z.T
+    return 0.5 * (h + h.T), lam, rng.normal(size=n)
+
+
+def test_lanczos_reorth_orthogonal_ritz_exact_no_ghosts(O):
+    """Full reorthogonalisation (no NanoGW counterpart, textbook scheme):
+    Q^T Q = I and Q^T H Q = T (a on the diagonal, b(k+1) coupling q_k and
+    q_k+1) at niter = n; the Ritz values are the eigenvalues, each once; the
+    continued fraction is the exact resolvent.  The same run without
+    reorthogonalisation loses orthogonality (> 0.1) and repeats the top
+    eigenvalue, so the case tests what it claims."""
+    n = 60
+    h, lam, v = _separated_extremes(n, 5)
+    vn = v / np.linalg.norm(v)
+    zz = np.array([0.5 + 0.05j, 3.0 + 0.01j, 7.0 + 0.2j, -0.5 + 0.1j])
+    spec, a, b, Q = O.lanczos_reorth(lambda x: h @ x, v, zz, n, reorth=True)
+    T = np.diag(a) + np.diag(b[1:], 1) + np.diag(b[1:], -1)
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-13
+    assert np.abs(Q.T @ h @ Q - T).max() < 1e-12
+    assert np.abs(np.linalg.eigvalsh(T) - np.sort(lam)).max() < 1e-12
+    w, Z = np.linalg.eigh(h)
+    exact = (((Z.T @ vn) ** 2)[None, :] / (zz[:, None] - w[None, :])).sum(1)
+    assert rel_err(spec, exact) < 1e-10
+    _, a0, b0, Q0 = O.lanczos_reorth(lambda x: h @ x, v, zz, n, reorth=False)
+    assert np.abs(Q0.T @ Q0 - np.eye(n)).max() > 0.1
+    T0 = np.diag(a0) + np.diag(b0[1:], 1) + np.diag(b0[1:], -1)
+    assert np.sum(np.abs(np.linalg.eigvalsh(T0) - 8.0) < 1e-8) > 1
+
+
+def test_lanczos_reorth_recurrence_is_the_reference_one(O):
+    """Before orthogonality is lost the reorthogonalised run equals the plain
+    one, and the plain numpy restatement equals the C restatement of
+    lanczos_spectra_BLOCK.F90 (same recurrence order and b(1) = 1)."""
+    h, _, v = _separated_extremes(40, 8)
+    zz = np.array([0.5 + 0.05j, 3.0 + 0.01j])
+    s1, a1, b1, _ = O.lanczos_reorth(lambda x: h @ x, v, zz, 6, reorth=True)
+    s0, a0, b0, _ = O.lanczos_reorth(lambda x: h @ x, v, zz, 6, reorth=False)
+    sc, ac, bc = O.lanczos_dense(h, v, zz, 6)
+    assert np.abs(a1 - a0).max() < 1e-12 and np.abs(b1 - b0).max() < 1e-12
+    assert np.abs(a0 - ac).max() < 1e-12 and np.abs(b0 - bc).max() < 1e-12
+    assert rel_err(s0, sc) < 1e-12 and rel_err(s1, sc) < 1e-11
+
+
 # --------------------------------------------------------------- operator
 @pytest.mark.parametrize("tda", [False, True])
 def test_operator_dense_closed_form(O, tiny, tda):
