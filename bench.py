@@ -392,6 +392,63 @@ def run_stream(args):
     ctx.close()
 
 
+def run_single_vector(args):
+    """One-vector FP64 operator applications on the R = 13 molecule, the call
+    shape of NanoGW's single-vector routines (matvec_isdf,
+    apply_matvec.f90:25; the UltraLowMem variants, apply_matvec_UltraLowMem.F90:25):
+    the factored form (ngw_set_isdf) and the explicit form (ngw_set_explicit_c
+    with C built from the same orbitals).  Each form: one untimed application,
+    then --sv-reps applications between two stream synchronisations; the line
+    gives microseconds per application and the rate over the form's
+    algorithmic bytes (factored 8 (Nmu nb + Nmu^2 + 3 ncv), explicit
+    8 (2 Nmu ncv + Nmu^2 + 3 ncv))."""
+    import torch
+
+    from nanogw_amd.lanczos import IsdfContext
+
+    ctx = IsdfContext(0)
+    mol, _, _ = build_workload(args.config, ctx)
+    nv, nc, nmu, nb = mol.nv, mol.nc, mol.n_intp, mol.nb
+    ncv = nv * nc
+    x = torch.randn(ncv, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    out = {"metric": "one-vector FP64 operator apply", "unit": "us per application", "higher_is_better": False,
+           "n_gpus": 1, "dtype": "f64", "data": "synthetic (seeded Gaussian-Hermite molecule)",
+           "config": {"workload": f"{args.config}: Nmu={nmu} nv={nv} nc={nc}, one vector per call",
+                      "reps": args.sv_reps}}
+
+    def timed():
+        torch.cuda.synchronize()
+        ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, 1)
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.sv_reps):
+            ctx.matvec_dev(x.data_ptr(), y.data_ptr(), ncv, 1)
+        ctx.synchronize()
+        return (time.perf_counter() - t) / args.sv_reps
+
+    ctx.set_isdf(mol.psi_intp, mol.mmtrx, mol.e1, nv)
+    ctx.set_precision("f64")
+    dt = timed()
+    nbytes = 8.0 * (nmu * nb + nmu * nmu + 3 * ncv)
+    out["factored"] = {"us": dt * 1e6, "bytes": nbytes, "TB_per_s": nbytes / dt / 1e12,
+                       "gflops": (4.0 * nmu * ncv + 2.0 * nmu * nmu) / dt / 1e9}
+    log(f"[bench] one vector, factored: {dt * 1e6:.1f} us ({nbytes / dt / 1e12:.2f} TB/s)")
+    if not args.sv_no_explicit:
+        psi = mol.psi_intp
+        # C(mu, icv) = Psi_c(mu) Psi_v(mu), icv = v + nv c (column-major)
+        cm = np.asfortranarray((psi[:, nv:nv + nc][:, :, None] * psi[:, :nv][:, None, :]).reshape(nmu, ncv))
+        ctx.set_explicit_c(cm, mol.mmtrx, np.sqrt((mol.e1[nv:][:, None] - mol.e1[:nv][None, :]).reshape(-1)))
+        del cm
+        dt = timed()
+        nbytes = 8.0 * (2 * nmu * ncv + nmu * nmu + 3 * ncv)
+        out["explicit"] = {"us": dt * 1e6, "bytes": nbytes, "TB_per_s": nbytes / dt / 1e12}
+        log(f"[bench] one vector, explicit C: {dt * 1e6:.1f} us ({nbytes / dt / 1e12:.2f} TB/s)")
+    out["value"] = out["factored"]["us"]
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -409,7 +466,10 @@ def main():
     ap.add_argument("--duration", type=float, default=5.0, help="c5: seconds of arrivals")
     ap.add_argument("--large-frac", type=float, default=0.1, help="c5: share of R = 13, nv = 100 requests")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch each step directly")
-    ap.add_argument("--mode", default="served", choices=["served", "single"])
+    ap.add_argument("--mode", default="served", choices=["served", "single", "single-vector"],
+                    help="served / single: the QP-solve headline; single-vector: one-vector FP64 operator timing")
+    ap.add_argument("--sv-reps", type=int, default=200, help="single-vector: timed applications per form")
+    ap.add_argument("--sv-no-explicit", action="store_true", help="single-vector: skip the explicit-C form")
     ap.add_argument("--precision", default="f16", choices=["f64", "f32", "bf16", "bf16x3", "f16"])
     ap.add_argument("--graph", action="store_true",
                     help="served mode: replay each Lanczos step as a hipGraph (no per-kernel timers)")
@@ -438,6 +498,9 @@ def main():
         return
     if args.config == "c5":
         run_stream(args)
+        return
+    if args.mode == "single-vector":
+        run_single_vector(args)
         return
 
     import torch

@@ -292,12 +292,14 @@ int ngw_gemm_f64_dev(ngw_context* ctx, int m, int n, int k, const double* d_a, l
  * Tile-shape autotuner of the low-precision operator GEMMs (no NanoGW
  * counterpart: NanoGW calls vendor BLAS).  For the context's current
  * precision and operator sizes, applies the operator to nvec vectors with
- * each of the four {BM, BN, slabs-in-flight} configurations of K1 (C x) and
- * K3 (C^T w), reps timed applications each, keeps the fastest of each for
- * later calls on this context (and batches created from it) and returns the
- * chosen indices.  A no-op in FP64 mode.  ngw_set_tile_config sets the
- * indices (0..3) directly; index 0 is the default.
+ * each of the NGW_NTILE_CONFIGS {BM, BN, slabs-in-flight, XCD M-groups}
+ * configurations of K1 (C x) and K3 (C^T w), reps timed applications each,
+ * keeps the fastest of each for later calls on this context (and batches
+ * created from it) and returns the chosen indices.  A no-op in FP64 mode.
+ * ngw_set_tile_config sets the indices (0 .. NGW_NTILE_CONFIGS - 1)
+ * directly; index 0 is the default.
  */
+#define NGW_NTILE_CONFIGS 6
 int ngw_autotune(ngw_context* ctx, int nvec, int reps, int* k1_cfg, int* k3_cfg);
 int ngw_set_tile_config(ngw_context* ctx, int k1_cfg, int k3_cfg);
 

@@ -239,7 +239,7 @@ int ngw_destroy(ngw_context* ctx)
                       &ctx->tpart, &ctx->ubuf, &ctx->w0, &ctx->w1, &ctx->w2, &ctx->w3, &ctx->w4,
                       &ctx->ab, &ctx->zz, &ctx->spec, &ctx->misc, &ctx->misc2, &ctx->staging,
                       &ctx->pct32, &ctx->pc32, &ctx->pctb, &ctx->pcb, &ctx->pcth, &ctx->pch, &ctx->m32t,
-                      &ctx->psiv32, &ctx->psivt, &ctx->psiv_scale, &ctx->xsig, &ctx->lpwork, &ctx->gv_part};
+                      &ctx->psiv32, &ctx->psiv_scale, &ctx->xsig, &ctx->dpairs, &ctx->lpwork, &ctx->gv_part};
     for (DevBuf* b : bufs) b->release();
     explicit_release(ctx);
     for (auto& r : ctx->prof_recs) {
@@ -379,7 +379,7 @@ int ngw_autotune(ngw_context* ctx, int nvec, int reps, int* k1_cfg, int* k3_cfg)
     a.x = x.as<double>(); a.ldx = ctx->ncv; a.nvec = nvec; a.out = y.as<double>(); a.ldo = ctx->ncv;
     double best1 = 1e300, best3 = 1e300;
     int b1 = 0, b3 = 0, st = NGW_OK;
-    for (int c = 0; c < 4 && st == NGW_OK; ++c) {
+    for (int c = 0; c < NGW_NTILE_CONFIGS && st == NGW_OK; ++c) {
         ctx->lp_cfg[prec][0] = c;
         ctx->lp_cfg[prec][1] = c;
         st = operator_apply(ctx, a);  // first use of a configuration
@@ -405,7 +405,8 @@ int ngw_autotune(ngw_context* ctx, int nvec, int reps, int* k1_cfg, int* k3_cfg)
 int ngw_set_tile_config(ngw_context* ctx, int k1_cfg, int k3_cfg)
 {
     if (!ctx) return fail(NGW_ERR_ARG, "null context");
-    if (k1_cfg < 0 || k1_cfg > 3 || k3_cfg < 0 || k3_cfg > 3) return fail(NGW_ERR_ARG, "configuration index 0..3");
+    if (k1_cfg < 0 || k1_cfg >= NGW_NTILE_CONFIGS || k3_cfg < 0 || k3_cfg >= NGW_NTILE_CONFIGS)
+        return fail(NGW_ERR_ARG, "configuration index 0.." + std::to_string(NGW_NTILE_CONFIGS - 1));
     if (ctx->precision == PREC_F64) return NGW_OK;
     ctx->lp_cfg[ctx->precision][0] = k1_cfg;
     ctx->lp_cfg[ctx->precision][1] = k3_cfg;

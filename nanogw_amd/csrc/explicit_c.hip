@@ -4,16 +4,14 @@
 // tmpCmtrx(n_intp, ncv) with C(mu, icv) = Psi_v(mu) Psi_c(mu), and
 //   y = R^3/2 (R^1/2 x) + 4 R^1/2 C^T M C (R^1/2 x)       (full, :126-131)
 //   y = R x + 2 C^T M C x                                 (TDA)
-// Also the factored operator on one vector (operator_f64_one, below).
-// The three products are library GEMMs (rocBLAS dgemm) for several vectors;
-// for one vector (the reference's three dgemv calls) C x and M t run on the
-// column-split GEMV kernels below and C^T u on rocBLAS dgemv.  The diagonal
-// term and the Horner / Lanczos combination are the same epilogue as the
-// factored operator.  FP64 only.
+// All products are this library's FP64 kernels: for one vector (the
+// reference's three dgemv calls) C x and M t on the column-split GEMV and
+// C^T u on a one-wave-per-column GEMV; for several vectors the FP64 MFMA
+// GEMMs (C x and M t with their sums split into ranges, the range partials
+// added in order).  The diagonal term and the Horner / Lanczos combination
+// are the same epilogue as the factored operator.  FP64 only.
 
 #include "ngw_kernels.h"
-
-#include <rocblas/rocblas.h>
 
 #include <cmath>
 #include <vector>
@@ -108,41 +106,28 @@ int gemv_wide(hipStream_t s, const double* A, long lda, int m, int n, const doub
     return NGW_OK;
 }
 
-namespace {
-
-int rb(rocblas_status st, const char* what)
+// y(j) = sum_i A(i, j) x(i) for column-major A (m x n), one wave per column
+// (a coalesced run down the column), lanes combined by a fixed butterfly.
+__global__ __launch_bounds__(256) void gemv_t_kernel(const double* A, long lda, int m, long n, const double* x,
+                                                     double* y)
 {
-    if (st != rocblas_status_success)
-        return fail(NGW_ERR_HIP, std::string(what) + " failed (rocblas status " + std::to_string((int)st) + ")");
-    return NGW_OK;
-}
-
-// The context's rocBLAS handle (created on first use) bound to its stream.
-// y(idx) = sum of the S chunk products (chunk order), then the epilogue
-__global__ void chunk_sum_epilogue_kernel(OperatorEpilogue ep, const double* part, int S, long ncv)
-{
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ncv; i += (long)gridDim.x * blockDim.x) {
-        double y = 0.0;
-        for (int c = 0; c < S; ++c) y += part[(long)c * ncv + i];
-        ep.apply(i, 0, y);
+    const int lane = threadIdx.x & 63;
+    const long nw = (long)gridDim.x * 4;
+    for (long j = blockIdx.x * 4L + (threadIdx.x >> 6); j < n; j += nw) {
+        const double* a = A + j * lda;
+        double s0 = 0.0, s1 = 0.0;
+        int i = lane;
+        for (; i + 64 < m; i += 128) {
+            s0 += a[i] * x[i];
+            s1 += a[i + 64] * x[i + 64];
+        }
+        if (i < m) s0 += a[i] * x[i];
+        double t = s0 + s1;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) y[j] = t;
     }
 }
-
-rocblas_handle blas(ngw_context* ctx, int& st)
-{
-    st = NGW_OK;
-    if (!ctx->rb_handle) {
-        rocblas_handle h = nullptr;
-        st = rb(rocblas_create_handle(&h), "rocblas_create_handle");
-        if (st != NGW_OK) return nullptr;
-        ctx->rb_handle = h;
-    }
-    rocblas_handle h = static_cast<rocblas_handle>(ctx->rb_handle);
-    st = rb(rocblas_set_stream(h, ctx->stream), "rocblas_set_stream");
-    return h;
-}
-
-}  // namespace
 
 int operator_explicit(ngw_context* ctx, const OperatorArgs& a)
 {
@@ -152,9 +137,6 @@ int operator_explicit(ngw_context* ctx, const OperatorArgs& a)
     hipStream_t s = ctx->stream;
     const long ncv = ctx->ncv;
     const int nmu = ctx->n_intp;
-    int st = NGW_OK;
-    rocblas_handle h = blas(ctx, st);
-    NGW_TRY(st);
     NGW_TRY(ctx->ex_x.ensure(sizeof(double) * (size_t)ncv * B));
     NGW_TRY(ctx->ex_y.ensure(sizeof(double) * (size_t)ncv * B));
     NGW_TRY(ctx->ex_t.ensure(sizeof(double) * 2 * (size_t)nmu * B));
@@ -165,120 +147,25 @@ int operator_explicit(ngw_context* ctx, const OperatorArgs& a)
     hipLaunchKernelGGL(scale_in_kernel, dim3(2048), dim3(256), 0, s, a.x, a.ldx,
                        a.tda ? nullptr : ctx->sqrtr.as<double>(), ncv, B, xs);
     NGW_HIP(hipGetLastError());
-    const double one = 1.0, zero = 0.0;
     const double* C = ctx->cexp.as<double>();
-    // t = C xs, u = M t, y = C^T u: GEMV for one vector (the library's GEMM
-    // kernels tile n = 1 as a full 128-wide block; its plain GEMV splits only
-    // the n_intp rows of the wide C x over workgroups), GEMM otherwise
     const double* M = ctx->mmat.as<double>();
     if (B == 1) {
         NGW_TRY(gemv_wide(s, C, nmu, nmu, (int)ncv, xs, t, ctx->gv_part));
         NGW_TRY(gemv_wide(s, M, nmu, nmu, nmu, t, u, ctx->gv_part));
-        NGW_TRY(rb(rocblas_dgemv(h, rocblas_operation_transpose, nmu, (int)ncv, &one, C, nmu, u, 1, &zero, y, 1),
-                   "dgemv C^T u"));
+        hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)std::min<long>((ncv + 3) / 4, 4096)), dim3(256), 0, s, C,
+                           (long)nmu, nmu, ncv, u, y);
     } else {
-        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nmu, B, (int)ncv, &one, C, nmu,
-                                 xs, (int)ncv, &zero, t, nmu), "dgemm C x"));
-        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nmu, B, nmu, &one, M, nmu, t,
-                                 nmu, &zero, u, nmu), "dgemm M t"));
-        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (int)ncv, B, nmu, &one, C,
-                                 nmu, u, nmu, &zero, y, (int)ncv), "dgemm C^T u"));
+        NGW_TRY(gemm_f64_split(s, nmu, B, (int)ncv, C, nmu, xs, ncv, t, nmu, ctx->ex_s));
+        NGW_TRY(gemm_f64_split(s, nmu, B, nmu, M, nmu, t, nmu, u, nmu, ctx->ex_s));
+        NGW_TRY(gemm_f64_tn(s, (int)ncv, B, nmu, 1.0, C, nmu, u, nmu, y, ncv));
     }
     hipLaunchKernelGGL(explicit_epilogue_kernel, dim3(2048), dim3(256), 0, s, a.epilogue(ctx), y, ncv, B);
     NGW_HIP(hipGetLastError());
     return NGW_OK;
 }
 
-namespace {
-
-// t(mu) = sum_v Psi_v(mu, v) T1(mu, v)
-__global__ void rowdot_kernel(const double* psi, long ldp, const double* t1, int nmu, int nv, double* t)
-{
-    const int mu = blockIdx.x * 256 + threadIdx.x;
-    if (mu >= nmu) return;
-    double s = 0.0;
-    for (int v = 0; v < nv; ++v) s += psi[mu + (long)v * ldp] * t1[mu + (long)v * nmu];
-    t[mu] = s;
-}
-
-// W(mu, v) = Psi_v(mu, v) u(mu)
-__global__ void weight_kernel(const double* psi, long ldp, const double* u, int nmu, int nv, double* w)
-{
-    const long n = (long)nmu * nv;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int v = (int)(e / nmu), mu = (int)(e - (long)v * nmu);
-        w[e] = psi[mu + (long)v * ldp] * u[mu];
-    }
-}
-
-}  // namespace
-
-// The factored operator on one vector.  The MFMA tiles of operator_f64 would
-// run nearly empty at N = nv columns, so the products are regrouped into
-// library GEMMs with the pair index split as (v, c), icv = v + nv c:
-//   T1 = Psi_c X^T          (n_intp x nv;  X(v, c) = (s o x)(icv))
-//   t(mu) = sum_v Psi_v(mu, v) T1(mu, v)             (= C (s o x))
-//   u = M t                                           (column-split GEMV)
-//   Y^T = (Psi_v o u)^T Psi_c   (nv x nc, Y^T(v, c) = (C^T u)(icv)),
-//         split over n_intp into S chunks (one strided-batched GEMM: the
-//         nv x nc output alone is too few tiles to fill the GPU)
-// then the chunk sum and the shared epilogue.
-int operator_f64_one(ngw_context* ctx, const OperatorArgs& a)
-{
-    hipStream_t s = ctx->stream;
-    const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond;
-    const long ncv = ctx->ncv, ldp = ctx->ld_psi;
-    int st = NGW_OK;
-    rocblas_handle h = blas(ctx, st);
-    NGW_TRY(st);
-    NGW_TRY(ctx->ex_x.ensure(sizeof(double) * (size_t)ncv));
-    NGW_TRY(ctx->ex_t.ensure(sizeof(double) * (2 * (size_t)nmu * nv + 2 * (size_t)nmu)));
-    double* xs = ctx->ex_x.as<double>();
-    double* t1 = ctx->ex_t.as<double>();  // n_intp x nv
-    double* t = t1 + (size_t)nmu * nv;
-    double* u = t + nmu;
-    double* w = u + nmu;
-    const double* psi = ctx->psi.as<double>();
-    const double* psic = psi + (size_t)nv * ldp;
-    const double one = 1.0, zero = 0.0;
-    hipLaunchKernelGGL(scale_in_kernel, dim3(1024), dim3(256), 0, s, a.x, a.ldx,
-                       a.tda ? nullptr : ctx->sqrtr.as<double>(), ncv, 1, xs);
-    {
-        ProfScope ps(ctx, KC_K1);
-        NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nmu, nv, nc, &one, psic,
-                                 (int)ldp, xs, nv, &zero, t1, nmu), "dgemm Psi_c X^T"));
-        hipLaunchKernelGGL(rowdot_kernel, dim3((nmu + 255) / 256), dim3(256), 0, s, psi, ldp, t1, nmu, nv, t);
-    }
-    {
-        ProfScope ps(ctx, KC_K2);
-        NGW_TRY(gemv_wide(s, ctx->mmat.as<double>(), ctx->ld_m, nmu, nmu, t, u, ctx->gv_part));
-    }
-    {
-        ProfScope ps(ctx, KC_K3);
-        hipLaunchKernelGGL(weight_kernel, dim3(std::min((nmu * (long)nv + 255) / 256, 2048L)), dim3(256), 0, s, psi,
-                           ldp, u, nmu, nv, w);
-        const int S = std::max(1, std::min(16, nmu / 256));
-        const int kc = nmu / S, rem = nmu - S * kc;
-        NGW_TRY(ctx->ex_s.ensure(sizeof(double) * (size_t)S * ncv));
-        double* part = ctx->ex_s.as<double>();
-        NGW_TRY(rb(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, nv, nc, kc,
-                                                 &one, w, nmu, (rocblas_stride)kc, psic, (int)ldp, (rocblas_stride)kc,
-                                                 &zero, part, nv, (rocblas_stride)ncv, S),
-                   "dgemm W^T Psi_c (chunks)"));
-        if (rem > 0)
-            NGW_TRY(rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, nc, rem, &one,
-                                     w + (size_t)S * kc, nmu, psic + (size_t)S * kc, (int)ldp, &one,
-                                     part + (size_t)(S - 1) * ncv, nv), "dgemm W^T Psi_c (rest)"));
-        hipLaunchKernelGGL(chunk_sum_epilogue_kernel, dim3(1024), dim3(256), 0, s, a.epilogue(ctx), part, S, ncv);
-    }
-    NGW_HIP(hipGetLastError());
-    return NGW_OK;
-}
-
 void explicit_release(ngw_context* ctx)
 {
-    if (ctx->rb_handle) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->rb_handle));
-    ctx->rb_handle = nullptr;
     ctx->cexp.release();
     ctx->ex_x.release();
     ctx->ex_y.release();

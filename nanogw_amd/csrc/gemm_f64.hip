@@ -58,6 +58,64 @@ int gemm_f64_tn(hipStream_t s, int m, int n, int k, double alpha, const double* 
     return NGW_OK;
 }
 
+// C = A B (column-major A m x k, B k x n) with the k sum split into ranges
+// when the tiles alone are too few for the GPU (n of a few vectors): grid
+// tiles x S, range partials added in range order by split_sum_kernel.
+constexpr int GSM = 128, GSN = 64;
+__global__ __launch_bounds__(GNT) void gemm_split_kernel(const double* a, long lda, const double* b, long ldb, int m,
+                                                         int n, int k, int kc, int mtiles, int ntiles, double* part)
+{
+    __shared__ SmemF64<GSM, GSN> sm;
+    const int split = blockIdx.y;
+    const int tile = xcd_tile(blockIdx.x, mtiles * ntiles);
+    const int m0 = (tile % mtiles) * GSM, n0 = (tile / mtiles) * GSN;
+    const int k0 = split * kc, K = min(kc, k - k0);
+    AccF64<GSM, GSN> acc;
+    gemm_f64_tile<GSM, GSN>(ColMajorA{a + (long)k0 * lda, lda, m}, ColMajorB{b + k0, ldb, n}, m0, n0, K, acc, sm);
+    const long mn = (long)m * n;
+    for_acc_f64<GSM, GSN>(acc, [&](int row, int col, double val) {
+        const int gm = m0 + row, gn = n0 + col;
+        if (gm < m && gn < n) part[(long)split * mn + gm + (long)gn * m] = val;
+    });
+}
+
+__global__ void split_sum_kernel(const double* part, int S, int m, int n, double* c, long ldc)
+{
+    const long mn = (long)m * n;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < mn; t += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int p = 0; p < S; ++p) s += part[(long)p * mn + t];
+        const long gn = t / m, gm = t - gn * m;
+        c[gm + gn * ldc] = s;
+    }
+}
+
+// Splits of a sum of length k over `tiles` tiles: about 1024 workgroups in
+// all, each range at least 64 long (four 16-deep slabs).
+static int sum_splits(int tiles, int k)
+{
+    const int want = std::max(1, 1024 / std::max(tiles, 1));
+    return std::max(1, std::min(want, (k + 63) / 64));
+}
+
+int gemm_f64_split(hipStream_t s, int m, int n, int k, const double* a, long lda, const double* b, long ldb,
+                   double* c, long ldc, DevBuf& part)
+{
+    if (m <= 0 || n <= 0) return NGW_OK;
+    const int mt = (m + GSM - 1) / GSM, nt = (n + GSN - 1) / GSN;
+    int S = sum_splits(mt * nt, k);
+    const int kc = (k + S - 1) / S;
+    S = (k + kc - 1) / kc;
+    NGW_TRY(part.ensure(sizeof(double) * (size_t)S * m * n));
+    hipLaunchKernelGGL(gemm_split_kernel, dim3(mt * nt, S), dim3(GNT), 0, s, a, lda, b, ldb, m, n, k, kc, mt, nt,
+                       part.as<double>());
+    const long mn = (long)m * n;
+    hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)std::min<long>((mn + 255) / 256, 4096)), dim3(256), 0, s,
+                       part.as<const double>(), S, m, n, c, ldc);
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
+}
+
 // ---------------------------------------------------------------- K1
 constexpr int K1M = 128, K1N = 64;
 
@@ -155,6 +213,132 @@ __global__ __launch_bounds__(GNT) void k3_f64_kernel(ColMajorAT al, WeightedPsiV
             ep.apply(v + (long)c * nv, b, val);
         }
     });
+}
+
+// ---------------------------------------------------------------- one vector
+// The operator on one vector (NanoGW's single-vector routines,
+// apply_matvec_UltraLowMem.F90:25-222): K1 and K3 have only N = nv columns,
+// far too few tiles to fill the GPU, so the sum of each tile is split into
+// S ranges (grid = tiles x S); a second pass adds the S range partials in
+// range order -- no atomics, so the result does not depend on scheduling.
+constexpr int OV1M = 128, OV1N = 64;  // K1: mu x v tile, sum over c
+constexpr int OV3M = 32, OV3N = 128;  // K3: c x v tile, sum over mu
+
+// part[(split ntn + tn) nmu + mu] = sum_{v in tile} PsiV(mu, v) sum_{c in range} PsiC(mu, c) xs(v, c)
+__global__ __launch_bounds__(GNT) void ov_k1_kernel(const double* psi, long ldp, const double* xs, int nmu, int nc,
+                                                    int nv, int kc, int mtiles, int ntn, double* part)
+{
+    __shared__ union {
+        SmemF64<OV1M, OV1N> ab;
+        double red[OV1M][OV1N + 2];
+    } sm;
+    const int split = blockIdx.y;
+    const int tile = xcd_tile(blockIdx.x, mtiles * ntn);
+    const int tm = tile % mtiles, tn = tile / mtiles;
+    const int m0 = tm * OV1M, n0 = tn * OV1N;
+    const int c0 = split * kc, K = min(kc, nc - c0);
+    AccF64<OV1M, OV1N> acc;
+    gemm_f64_tile<OV1M, OV1N>(ColMajorA{psi + (long)(nv + c0) * ldp, ldp, nmu},
+                              XprimeB{xs + (long)c0 * nv, nullptr, 0, nv, nv}, m0, n0, K, acc, sm.ab);
+    for_acc_f64<OV1M, OV1N>(acc, [&](int row, int col, double val) {
+        const int gm = m0 + row, v = n0 + col;
+        sm.red[row][col] = (gm < nmu && v < nv) ? val * psi[gm + (long)v * ldp] : 0.0;
+    });
+    __syncthreads();
+    if (threadIdx.x < OV1M) {
+        const int row = threadIdx.x, gm = m0 + row;
+        double s = 0.0;
+        for (int cc = 0; cc < OV1N; ++cc) s += sm.red[row][cc];
+        if (gm < nmu) part[((long)split * ntn + tn) * nmu + gm] = s;
+    }
+}
+
+// t(mu) = sum over the nparts partials, in order
+__global__ void ov_sum_kernel(const double* part, int nparts, long n, double* t)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int p = 0; p < nparts; ++p) s += part[(long)p * n + i];
+        t[i] = s;
+    }
+}
+
+// part[split ncv + v + nv c] = sum_{mu in range} PsiC(mu, c) PsiV(mu, v) u(mu)
+__global__ __launch_bounds__(GNT) void ov_k3_kernel(const double* psi, long ldp, const double* u, int nmu, int nc,
+                                                    int nv, int kmu, int mtiles, int ntn, double* part)
+{
+    __shared__ SmemF64<OV3M, OV3N> sm;
+    const int split = blockIdx.y;
+    const int tile = xcd_tile(blockIdx.x, mtiles * ntn);
+    const int m0 = (tile % mtiles) * OV3M, n0 = (tile / mtiles) * OV3N;
+    const int mu0 = split * kmu, K = min(kmu, nmu - mu0);
+    AccF64<OV3M, OV3N> acc;
+    gemm_f64_tile<OV3M, OV3N>(ColMajorAT{psi + (long)nv * ldp + mu0, ldp, nc},
+                              WeightedPsiVB{psi + mu0, ldp, u + mu0, 0, nv, nv}, m0, n0, K, acc, sm);
+    const long ncv = (long)nv * nc;
+    for_acc_f64<OV3M, OV3N>(acc, [&](int row, int col, double val) {
+        const int c = m0 + row, v = n0 + col;
+        if (c < nc && v < nv) part[(long)split * ncv + v + (long)c * nv] = val;
+    });
+}
+
+// y(idx) = sum of the S partials (range order), then the operator epilogue
+__global__ void ov_epilogue_kernel(OperatorEpilogue ep, const double* part, int S, long ncv)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ncv; i += (long)gridDim.x * blockDim.x) {
+        double y = 0.0;
+        for (int c = 0; c < S; ++c) y += part[(long)c * ncv + i];
+        ep.apply(i, 0, y);
+    }
+}
+
+__global__ void ov_scale_kernel(const double* x, const double* s, long ncv, double* xs)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ncv; i += (long)gridDim.x * blockDim.x)
+        xs[i] = s ? s[i] * x[i] : x[i];
+}
+
+int operator_f64_one(ngw_context* ctx, const OperatorArgs& a)
+{
+    hipStream_t s = ctx->stream;
+    const int nmu = ctx->n_intp, nv = ctx->nval, nc = ctx->ncond;
+    const long ncv = ctx->ncv, ldp = ctx->ld_psi;
+    const double* psi = ctx->psi.as<double>();
+    const int mt1 = (nmu + OV1M - 1) / OV1M, nt1 = (nv + OV1N - 1) / OV1N;
+    const int mt3 = (nc + OV3M - 1) / OV3M, nt3 = (nv + OV3N - 1) / OV3N;
+    int S1 = sum_splits(mt1 * nt1, nc), S3 = sum_splits(mt3 * nt3, nmu);
+    const int kc = (nc + S1 - 1) / S1, kmu = (nmu + S3 - 1) / S3;
+    S1 = (nc + kc - 1) / kc;
+    S3 = (nmu + kmu - 1) / kmu;
+    NGW_TRY(ctx->ex_x.ensure(sizeof(double) * (size_t)ncv));
+    NGW_TRY(ctx->ex_t.ensure(sizeof(double) * ((size_t)S1 * nt1 * nmu + 2 * (size_t)nmu)));
+    NGW_TRY(ctx->ex_s.ensure(sizeof(double) * (size_t)S3 * ncv));
+    double* xs = ctx->ex_x.as<double>();
+    double* p1 = ctx->ex_t.as<double>();
+    double* t = p1 + (size_t)S1 * nt1 * nmu;
+    double* u = t + nmu;
+    double* p3 = ctx->ex_s.as<double>();
+    const int gs = (int)std::min<long>((ncv + 255) / 256, 2048);
+    hipLaunchKernelGGL(ov_scale_kernel, dim3(gs), dim3(256), 0, s, a.x, a.tda ? nullptr : ctx->sqrtr.as<double>(),
+                       ncv, xs);
+    {
+        ProfScope ps(ctx, KC_K1);
+        hipLaunchKernelGGL(ov_k1_kernel, dim3(mt1 * nt1, S1), dim3(GNT), 0, s, psi, ldp, xs, nmu, nc, nv, kc, mt1,
+                           nt1, p1);
+        hipLaunchKernelGGL(ov_sum_kernel, dim3((nmu + 255) / 256), dim3(256), 0, s, p1, S1 * nt1, (long)nmu, t);
+    }
+    {
+        ProfScope ps(ctx, KC_K2);
+        NGW_TRY(gemv_wide(s, ctx->mmat.as<double>(), ctx->ld_m, nmu, nmu, t, u, ctx->gv_part));
+    }
+    {
+        ProfScope ps(ctx, KC_K3);
+        hipLaunchKernelGGL(ov_k3_kernel, dim3(mt3 * nt3, S3), dim3(GNT), 0, s, psi, ldp, u, nmu, nc, nv, kmu, mt3,
+                           nt3, p3);
+        hipLaunchKernelGGL(ov_epilogue_kernel, dim3(gs), dim3(256), 0, s, a.epilogue(ctx), p3, S3, ncv);
+    }
+    NGW_HIP(hipGetLastError());
+    return NGW_OK;
 }
 
 // ---------------------------------------------------------------- launchers

@@ -6,8 +6,7 @@
 //   per system (once):  pct = (Psi_c / sigma_c)^T  [mu][c]   K1 A operand
 //                       pc  =  Psi_c / sigma_c     [c][mu]   K3 A operand
 //                       m32t = M^T FP32            [mu][nu]  K2 A operand
-//                       psiv32 = Psi_v^T FP32      [v][mu]   wprep (K3 B factor)
-//                       psivt  = Psi_v FP32        [mu][v]   K1 epilogue
+//                       psiv32 = Psi_v^T FP32      [v][mu]   wprep (K3 B factor), K1 epilogue
 //   per operator application:
 //     xprep  x'(j, c) = sigma_c s(v,c) x_b(v,c)   (transpose through LDS)
 //     K1     T = pct x'^T, epilogue * Psi_v and reduce over v (LDS)
@@ -61,15 +60,23 @@ struct LpCfg {
     int fw = 0;  // K3 only: 1 = W formed in the slab loader (no wprep pass)
     int sw = 0;  // LDS layout (lds_chunk): 0 padded rows, 1 XOR-swizzled rows
 };
-constexpr int LP_NCFG = 4;
+constexpr int LP_NCFG = NGW_NTILE_CONFIGS;
 constexpr LpCfg K1_CFG[3][LP_NCFG] = {
-    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 128, 2, 4, 0, 1}, {256, 128, 2, 8, 0, 1}},  // FP32, BF16
-    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 64, 2, 8, 0, 1}, {128, 128, 1, 8, 0, 1}},   // BF16x3
-    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 128, 2, 4, 0, 1}, {256, 128, 2, 8, 0, 1}}}; // FP16
+    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 128, 2, 4, 0, 1}, {256, 128, 2, 8, 0, 1},
+     {128, 128, 2, 2, 0, 1}, {128, 128, 3, 8, 0, 1}},                                             // FP32, BF16
+    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 64, 2, 8, 0, 1}, {128, 128, 1, 8, 0, 1},
+     {128, 128, 2, 2, 0, 1}, {128, 64, 2, 2, 0, 1}},                                              // BF16x3
+    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 128, 2, 4, 0, 1}, {256, 128, 2, 8, 0, 1},
+     {128, 128, 2, 2, 0, 1}, {128, 128, 3, 8, 0, 1}}};                                            // FP16
+// PM = 1 or 2 (indices 4, 5): each XCD covers all or half of the c rows, so W
+// is read once or twice instead of once per M-group.
 constexpr LpCfg K3_CFG[3][LP_NCFG] = {
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {128, 128, 2, 8, 1, 1}},
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {64, 128, 2, 8}},
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {192, 128, 3, 8, 0, 1}, {128, 192, 2, 4, 0, 1}}};  // FP16: W from wprep
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {128, 128, 2, 8, 1, 1},
+     {128, 128, 2, 2, 0, 1}, {128, 128, 2, 1, 0, 1}},
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {64, 128, 2, 8},
+     {128, 128, 2, 2, 0, 1}, {128, 128, 2, 1, 0, 1}},
+    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {192, 128, 3, 8, 0, 1}, {128, 192, 2, 4, 0, 1},
+     {128, 192, 2, 2, 0, 1}, {128, 192, 2, 1, 0, 1}}};  // FP16: W from wprep
 constexpr int NXCD = 8;
 __host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : kind == LP_F16 ? 2 : 0; }
 static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : prec == PREC_F16 ? 2 : 0; }
@@ -147,14 +154,16 @@ __global__ void prep_m_kernel(const double* m, long ldm, int nmu, int ldm1, cons
     }
 }
 
-// psivt(mu, v) = Psi_v(mu, v) in FP32, v contiguous: the K1 epilogue reads
-// 16 consecutive v of one row per wave quarter instead of 16 rows.
-__global__ void psivt_kernel(const double* psi, long ldp, int nmu, int nv, float* psivt)
+// Diagonal factors of the K3 epilogue as FP32 pairs (d', s') per pair index:
+// full mode (d_full sqrtR, sqrtR), TDA (d_tda, 1), so that
+// hx = d' x + beta s' Y.  The low-precision modes carry x and the Clenshaw
+// intermediates in FP32, so FP32 factors add nothing to their error.
+__global__ void diag_pairs_kernel(const double* sqrtr, const double* dfull, const double* dtda, long ncv,
+                                  float2* full, float2* tda)
 {
-    const long n = (long)nmu * nv;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int mu = (int)(e / nv), v = (int)(e - (long)mu * nv);
-        psivt[e] = (float)psi[mu + (long)v * ldp];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < ncv; i += (long)gridDim.x * blockDim.x) {
+        full[i] = float2{(float)(dfull[i] * sqrtr[i]), (float)sqrtr[i]};
+        tda[i] = float2{(float)dtda[i], 1.0f};
     }
 }
 
@@ -214,7 +223,7 @@ int lp2_prepare(ngw_context* ctx)
     NGW_TRY(ctx->xsig.ensure(sizeof(float) * 2 * (size_t)ctx->ncv));
     NGW_TRY(ctx->m32t.ensure(4 * (size_t)(nmu + 1) * ctx->ldm1));
     NGW_TRY(ctx->psiv32.ensure(4 * (size_t)nv * ctx->ldm1));
-    NGW_TRY(ctx->psivt.ensure(4 * (size_t)nmu * nv));
+    NGW_TRY(ctx->dpairs.ensure(2 * sizeof(float2) * (size_t)ctx->ncv));
     hipStream_t s = ctx->stream;
     const double* psi = ctx->psi.as<double>();
     const float* sig = ctx->psi_scale.as<float>();
@@ -228,12 +237,13 @@ int lp2_prepare(ngw_context* ctx)
                        ctx->ldc1, ctx->ldm1, (void*)ctx->pcth.ptr, (void*)nullptr, (void*)ctx->pch.ptr, (void*)nullptr);
     hipLaunchKernelGGL(prep_m_kernel, dim3(2048), dim3(256), 0, s, ctx->mmat.as<double>(), (long)ctx->ld_m, nmu,
                        ctx->ldm1, psi, (long)ctx->ld_psi, nv, ctx->m32t.as<float>(), ctx->psiv32.as<float>());
-    hipLaunchKernelGGL(psivt_kernel, dim3(1024), dim3(256), 0, s, psi, (long)ctx->ld_psi, nmu, nv,
-                       ctx->psivt.as<float>());
     hipLaunchKernelGGL(psiv_scale_kernel, dim3(nv), dim3(256), 0, s, ctx->psiv32.as<float>(), ctx->ldm1, nmu,
                        ctx->psiv_scale.as<float>());
     hipLaunchKernelGGL(xsig_kernel, dim3(1024), dim3(256), 0, s, ctx->sqrtr.as<double>(), sig, nv, ctx->ncv,
                        ctx->xsig.as<float>());
+    hipLaunchKernelGGL(diag_pairs_kernel, dim3(1024), dim3(256), 0, s, ctx->sqrtr.as<double>(),
+                       ctx->diag_full.as<double>(), ctx->diag_tda.as<double>(), ctx->ncv, ctx->dpairs.as<float2>(),
+                       ctx->dpairs.as<float2>() + ctx->ncv);
     NGW_HIP(hipGetLastError());
     NGW_HIP(hipStreamSynchronize(s));
     ctx->lp2_ready = true;
@@ -534,16 +544,38 @@ __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, co
     const int nend = min(n0 + BN, N);
     const int b_lo = n0 / nv, b_hi = (nend - 1) / nv;
     const int nbt = b_hi - b_lo + 1;
+    // T(mu, j) Psi_v(mu, v(j)): a lane's accumulator fragment holds four
+    // consecutive mu of one column j, so Psi_v comes as one 16-byte load from
+    // the [v][mu] copy (zero for nmu <= mu < ldm1); v(j) once per column
+    constexpr int FM = BM / 32, FN = BN / 32;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
+    int vrow[FN];
+    bool cok[FN];
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+        const int gn = n0 + wn + 16 * j + (lane & 15);
+        cok[j] = gn < N;
+        vrow[j] = cok[j] ? gn % nv : 0;
+    }
 #pragma unroll
     for (int pass = 0; pass < BM / HALF; ++pass) {
         if (pass > 0) __syncthreads();
-        for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
-            if (row < pass * HALF || row >= (pass + 1) * HALF) return;
-            int gm = m0 + row, gn = n0 + col;
-            float p = 0.0f;
-            if (gm < op.nmu && gn < N) p = val * op.psivt[(long)gm * nv + gn % nv];
-            sm.red[row - pass * HALF][col] = p;
-        });
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int row0 = wm + 16 * i + 4 * (lane >> 4);  // a fragment lies in one pass
+            if (row0 < pass * HALF || row0 >= (pass + 1) * HALF) continue;
+            const int gm0 = m0 + row0;
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                f32x4 pv = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                if (cok[j] && gm0 < op.ldm1)
+                    pv = *reinterpret_cast<const f32x4*>(op.psiv32 + (long)vrow[j] * op.ldm1 + gm0);
+                const int col = wn + 16 * j + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.red[row0 + r - pass * HALF][col] = acc.v[i][j][r] * pv[r];
+            }
+        }
         __syncthreads();
         for (int e = threadIdx.x; e < HALF * nbt; e += GNT) {
             int row = e % HALF, b = b_lo + e / HALF;
@@ -723,8 +755,7 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
     // epilogue: OperatorEpilogue's stage (diagonal term, Horner / Clenshaw
     // combination, Lanczos accumulate), written out for one leading
     // dimension shared by every vector block of the request
-    const double* dd = op.tda ? op.dtda : op.dfull;
-    const double* ss = op.tda ? nullptr : op.sqrtr;
+    const float2* dp = op.dpair;
     const double beta = op.tda ? 2.0 : 4.0;
     const bool acc_on = sg.accumulate == 1 || (sg.accumulate == 2 && *op.kstep > 1);
     const bool use_q = op.q != nullptr && sg.gamma != 0.0;
@@ -741,8 +772,7 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
         // multiple of four (tiles start at multiples of four): then every
         // array is read and written with 16-byte accesses
         const bool vec = (nv & 3) == 0 && (op.ld & 3) == 0 && aligned16(xp) && aligned16(yp) && aligned16(op_) &&
-                         aligned16(use_q ? op.q : nullptr) && aligned16(dd) && aligned16(ss) &&
-                         (KIND != LP_F16 || aligned16(op.wsc));
+                         aligned16(use_q ? op.q : nullptr) && aligned16(dp) && (KIND != LP_F16 || aligned16(op.wsc));
         if (!vec) {
             for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
                 const int c = m0 + row, j = n0 + col;
@@ -751,9 +781,9 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                     const long idx = v + (long)c * nv, i = idx + (long)b * op.ld;
                     double Y = (double)val * (double)sigc[c];
                     if constexpr (KIND == LP_F16) Y *= (double)op.wsc[j] * F16_UNSCALE;
-                    const double sv = ss ? ss[idx] : 1.0;
+                    const float2 f = dp[idx];
                     const double xv = (double)xp[i];
-                    const double hx = dd[idx] * (ss ? sv * xv : xv) + (beta * sv) * Y;
+                    const double hx = (double)f.x * xv + (beta * (double)f.y) * Y;
                     double r = sg.alpha * hx + sg.bx * xv;
                     if (yp) r = r + sg.by * (double)yp[i];
                     if (use_q) r = r + sg.gamma * op.q[i];
@@ -807,15 +837,15 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                         const f32x4 w4 = *reinterpret_cast<const f32x4*>(op.wsc + j0);
                         ws[0] = w4.x; ws[1] = w4.y; ws[2] = w4.z; ws[3] = w4.w;
                     }
-                    double sv[4] = {1.0, 1.0, 1.0, 1.0};
                     ld4(xp + i, xv);
-                    ld4(dd + idx, t4);
-                    if (ss) ld4(ss + idx, sv);
+                    const f32x4 d01 = *reinterpret_cast<const f32x4*>(dp + idx);
+                    const f32x4 d23 = *reinterpret_cast<const f32x4*>(dp + idx + 2);
+                    const float dd4[4] = {d01.x, d01.z, d23.x, d23.z}, ss4[4] = {d01.y, d01.w, d23.y, d23.w};
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         double Y = (double)va[u] * sc;
                         if constexpr (KIND == LP_F16) Y *= (double)ws[u] * F16_UNSCALE;
-                        const double hx = t4[u] * (ss ? sv[u] * xv[u] : xv[u]) + (beta * sv[u]) * Y;
+                        const double hx = (double)dd4[u] * xv[u] + (beta * (double)ss4[u]) * Y;
                         res[u] = sg.alpha * hx + sg.bx * xv[u];
                     }
                 }
@@ -905,7 +935,9 @@ static int lp_launch(hipStream_t st, int c1, int c3, const LpOp* ops, const LpOp
             case 0: launch_k1<KIND, 0>(st, tot[2], ops, op1, pre[2], nreq); break;
             case 1: launch_k1<KIND, 1>(st, tot[2], ops, op1, pre[2], nreq); break;
             case 2: launch_k1<KIND, 2>(st, tot[2], ops, op1, pre[2], nreq); break;
-            default: launch_k1<KIND, 3>(st, tot[2], ops, op1, pre[2], nreq); break;
+            case 3: launch_k1<KIND, 3>(st, tot[2], ops, op1, pre[2], nreq); break;
+            case 4: launch_k1<KIND, 4>(st, tot[2], ops, op1, pre[2], nreq); break;
+            default: launch_k1<KIND, 5>(st, tot[2], ops, op1, pre[2], nreq); break;
         }
     }
     {
@@ -927,7 +959,9 @@ static int lp_launch(hipStream_t st, int c1, int c3, const LpOp* ops, const LpOp
             case 0: launch_k3<KIND, 0>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
             case 1: launch_k3<KIND, 1>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
             case 2: launch_k3<KIND, 2>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
-            default: launch_k3<KIND, 3>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
+            case 3: launch_k3<KIND, 3>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
+            case 4: launch_k3<KIND, 4>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
+            default: launch_k3<KIND, 5>(st, tot[6], ops, op1, stg, st1, nstage, s, pre[6], nreq); break;
         }
     }
     NGW_HIP(hipGetLastError());
@@ -982,11 +1016,11 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
         d.pct_hi = op->pctb.ptr; d.pct_lo = op->pctb.as<unsigned short>() + n1;
         d.pc_hi = op->pcb.ptr; d.pc_lo = op->pcb.as<unsigned short>() + n2;
     }
-    d.m32t = op->m32t.as<float>(); d.psiv32 = op->psiv32.as<float>(); d.psivt = op->psivt.as<float>();
+    d.m32t = op->m32t.as<float>(); d.psiv32 = op->psiv32.as<float>();
     d.psi = op->psi.as<double>(); d.ldp = op->ld_psi;
     d.sig = op->psi_scale.as<float>();
     d.xsig = op->xsig.as<float>() + (tda ? op->ncv : 0);
-    d.sqrtr = op->sqrtr.as<double>(); d.dfull = op->diag_full.as<double>(); d.dtda = op->diag_tda.as<double>();
+    d.dpair = op->dpairs.as<float2>() + (tda ? op->ncv : 0);
     d.nmu = op->n_intp; d.nv = op->nval; d.nc = op->ncond; d.nvec = nvec; d.ldc1 = op->ldc1; d.ldm1 = op->ldm1;
     d.ld = ld; d.tda = tda;
     const size_t N = (size_t)op->nval * nvec;

@@ -91,15 +91,15 @@ struct ngw_context {
     // pre-converted operands of the streamlined low-precision path
     bool lp2_ready = false;
     int ldc1 = 0, ldm1 = 0;
-    ngw::DevBuf pct32, pc32, pctb, pcb, pcth, pch, m32t, psiv32, psivt, psiv_scale, xsig, lpwork;
+    ngw::DevBuf pct32, pc32, pctb, pcb, pcth, pch, m32t, psiv32, psiv_scale, xsig, dpairs, lpwork;
     // chosen {K1, K3} tile configuration per precision mode (ngw_autotune)
     int lp_cfg[ngw::PREC_COUNT][2] = {};
     // explicit pair matrix C (n_intp x ncv, ngw_set_explicit_c) in place of
-    // the factored Psi_v o Psi_c; its rocBLAS handle and GEMM workspaces
+    // the factored Psi_v o Psi_c, and the workspaces of the explicit and
+    // one-vector products (scaled input, partial sums of split sums)
     bool explicit_c = false;
     ngw::DevBuf cexp, ex_x, ex_y, ex_t, ex_s;
     ngw::DevBuf gv_part;  // chunk sums of the one-vector GEMV (gemv_wide)
-    void* rb_handle = nullptr;
     // workspaces
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;
     // host staging for drop-in calls

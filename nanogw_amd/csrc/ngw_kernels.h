@@ -125,6 +125,10 @@ int pvectors_f64(ngw_context* ctx, const int* d_mband, const int* d_nband, int n
 // y = A x for a column-major m x n A, split over column chunks, summed in a
 // fixed order; part is the chunk-sum workspace.
 int gemv_wide(hipStream_t s, const double* A, long lda, int m, int n, const double* x, double* y, DevBuf& part);
+// C = A B (FP64, column-major) with the k sum split into ranges when n is a
+// few vectors; partials in `part`, added in range order.
+int gemm_f64_split(hipStream_t s, int m, int n, int k, const double* a, long lda, const double* b, long ldb,
+                   double* c, long ldc, DevBuf& part);
 // The operator with an explicit pair matrix C (rocBLAS products, FP64).
 int operator_explicit(ngw_context* ctx, const OperatorArgs& a);
 void explicit_release(ngw_context* ctx);
@@ -138,11 +142,10 @@ struct LpOp {
     const void* pct_hi; const void* pct_lo;
     const void* pc_hi; const void* pc_lo;
     const float* m32t; const float* psiv32;
-    const float* psivt;  // Psi_v FP32 [mu][v] (K1 epilogue: one row's v contiguous)
     const double* psi; long ldp;
     const float* sig;
     const float* xsig;  // x' = xsig o x: sigma_c sqrtR(v,c), or sigma_c for TDA (ncv)
-    const double* sqrtr; const double* dfull; const double* dtda;
+    const float2* dpair;  // K3 epilogue diagonal factors (d', s') of the request's mode (FP32)
     int nmu, nv, nc, nvec, ldc1, ldm1;
     long ld; int tda;
     void* xlp_hi; void* xlp_lo; void* wlp_hi; void* wlp_lo;
@@ -195,5 +198,6 @@ int sigma_assemble(hipStream_t s, const double* d_spec, const double* d_zz, cons
 int qp_newton(hipStream_t s, int nqp, int nen, double deltae_ev, const double* d_in,
               double* d_out, double* d_scratch);
 int sigma_x_isdf(ngw_context* ctx, int nqp, const int* d_qp, double* d_sigx);
+
 
 }  // namespace ngw

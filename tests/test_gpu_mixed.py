@@ -88,7 +88,7 @@ def test_tile_configs_and_autotune_vs_oracle(ngw_ctx, tall, prec, tol):
         x = np.random.default_rng(4).normal(size=(m.nv * m.nc, 37))
         ref = O.matvec_block(m.psi_intp, m.mmtrx, O.sqrt_r(m.e1, m.nv, m.nc), x, m.nv, m.nc)
         outs = []
-        for cfg in range(4):
+        for cfg in range(6):
             ngw_ctx.set_tile_config(cfg, cfg)
             y = ngw_ctx.matvec_isdf_BLOCK(x)
             err = max(rel_err(y[:, j], ref[:, j]) for j in range(x.shape[1]))
@@ -96,7 +96,7 @@ def test_tile_configs_and_autotune_vs_oracle(ngw_ctx, tall, prec, tol):
             assert err < tol, (cfg, err)
             outs.append(y)
         k1, k3 = ngw_ctx.autotune(37, reps=1)
-        assert 0 <= k1 < 4 and 0 <= k3 < 4
+        assert 0 <= k1 < 6 and 0 <= k3 < 6
         y = ngw_ctx.matvec_isdf_BLOCK(x)
         assert max(rel_err(y[:, j], ref[:, j]) for j in range(x.shape[1])) < tol
     finally:
