@@ -204,6 +204,29 @@ int ngw_qp_newton(ngw_context* ctx, int nqp, int nen, double deltae_ev, const do
                   const double* sigma_c, double* eqp, double* zfac, double* sigma0,
                   double* imsig, int* ierr);
 
+/* Largest nqp * nen of ngw_qp_newton_dev_noalloc (its work arrays take
+ * 5 * 8 bytes per point: 2.5 MiB per GPU). */
+#define NGW_QP_NOALLOC_MAX_POINTS 65536
+
+/*
+ * ngw_qp_newton on device-resident data, allocating nothing: no hipMalloc,
+ * hipFree or allocating library call; the spline work arrays live in a
+ * fixed per-GPU device array (NGW_QP_NOALLOC_MAX_POINTS points) that the
+ * library keeps until the process ends.  No counterpart in NanoGW: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).
+ *   d_in  device, 4 * nqp + 2 * nen * nqp doubles: (edft, en0, sigx, vxc) of
+ *         each state (eV), then sigma_c(nen, nqp) complex (eV)
+ *   d_out device, 6 * nqp doubles per state: eqp, zfac, Re sigma0,
+ *         Im sigma0, imsig, ierr (as a double; 0 = converged)
+ * Queued on the context's stream; the caller synchronises (ngw_synchronize).
+ * nqp * nen above the limit is NGW_ERR_ARG.  Every call on one GPU uses the
+ * same work arrays: calls that overlap on one GPU (several contexts or
+ * threads) race.
+ */
+int ngw_qp_newton_dev_noalloc(ngw_context* ctx, int nqp, int nen, double deltae_ev, const double* d_in,
+                              double* d_out);
+
 /*
  * Least-squares monomial fit of sqrt(x) on [xmin, xmax] through nx nodes,
  * replaces polyfit_sqrt (shared/polyfit.f90:1) and, with chebyshev = 1,

@@ -218,6 +218,7 @@ int ngw_create(ngw_context** out, int device)
     NGW_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(NGW_ERR_ARG, "device index out of range");
     NGW_HIP(hipSetDevice(device));
+    NGW_TRY(noalloc_setup(device));
     ngw_context* c = new ngw_context();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -198,6 +198,8 @@ int sigma_assemble(hipStream_t s, const double* d_spec, const double* d_zz, cons
 int qp_newton(hipStream_t s, int nqp, int nen, double deltae_ev, const double* d_in,
               double* d_out, double* d_scratch);
 int sigma_x_isdf(ngw_context* ctx, int nqp, const int* d_qp, double* d_sigx);
-
+// Per-device address of the no-alloc entry's work array (noalloc.hip);
+// called by ngw_create with the device current.
+int noalloc_setup(int device);
 
 }  // namespace ngw

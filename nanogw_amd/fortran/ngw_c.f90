@@ -136,6 +136,14 @@ module ngw_c
       complex(c_double_complex), intent(out) :: sigma0(*)
       integer(c_int), intent(out) :: ierr(*)
     end function ngw_qp_newton
+    ! no counterpart in NanoGW: device-resident inputs and outputs, allocates nothing
+    integer(c_int) function ngw_qp_newton_dev_noalloc(ctx, nqp, nen, deltae_ev, d_in, d_out) &
+      bind(C, name='ngw_qp_newton_dev_noalloc')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx, d_in, d_out
+      integer(c_int), value :: nqp, nen
+      real(c_double), value :: deltae_ev
+    end function ngw_qp_newton_dev_noalloc
 
     integer(c_int) function ngw_polyfit_sqrt(xmin, xmax, npoly, nx, chebyshev, beta, fit_error) &
       bind(C, name='ngw_polyfit_sqrt')
