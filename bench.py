@@ -259,6 +259,12 @@ def run_batched(args):
 
     ctx = IsdfContext(0)
     specs = synth.c4_specs(args.molecules)
+    lo, hi = 0, len(specs)
+    if args.subset:
+        lo, hi = (int(v) for v in args.subset.split(":"))
+        if not 0 <= lo < hi <= len(specs):
+            raise SystemExit(f"--subset {args.subset}: need 0 <= A < B <= {len(specs)}")
+        specs = specs[lo:hi]
     mols = []
     t = time.time()
     for i, sp in enumerate(specs):
@@ -302,7 +308,8 @@ def run_batched(args):
         "ms_per_step": 1000.0 * elapsed / len(mols), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": args.precision,
         "data": "synthetic (seeded Gaussian-Hermite molecules, CVT ISDF points, device V build)",
-        "config": {"workload": f"c4: {len(mols)} molecules, R~U[6,11] h=0.4 nv~U{{8..60}} nc=10nv Nmu=12nv, "
+        "config": {"workload": f"c4: {len(mols)} molecules (indices {lo}..{hi - 1} of the seeded draw of "
+                               f"{args.molecules}), R~U[6,11] h=0.4 nv~U{{8..60}} nc=10nv Nmu=12nv, "
                                f"16 QP states each, niter={niter}, npoly=12, nen={nen}; served/batched grouped "
                                f"launches" + ("" if args.no_graph else ", hipGraph step replay"),
                    "parallelism": "single GPU", "tile_configs": tiles,
@@ -462,6 +469,7 @@ def main():
                          "clamped to what fits")
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--molecules", type=int, default=64, help="c4 / c5: number of (catalog) molecules")
+    ap.add_argument("--subset", default="", help="c4: A:B runs molecules A..B-1 of the seeded draw only")
     ap.add_argument("--rate", type=float, default=20.0, help="c5: arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="c5: seconds of arrivals")
     ap.add_argument("--large-frac", type=float, default=0.1, help="c5: share of R = 13, nv = 100 requests")
@@ -483,6 +491,9 @@ def main():
     ap.add_argument("--band-chunk", type=int, default=0,
                     help="single mode: sum Sigma_c over band ranges of this many bands "
                          "(ngw_sigma_lanczos_bands), logging each range")
+    ap.add_argument("--reorth", action="store_true",
+                    help="single mode: full reorthogonalisation of the Lanczos recurrences (FP64; no NanoGW "
+                         "counterpart), to time its cost")
     ap.add_argument("--qp-states", default="",
                     help="comma-separated 0-based QP bands to time instead of HOMO, LUMO, HOMO-1, ...")
     ap.add_argument("--write-golden", action="store_true",
@@ -589,6 +600,17 @@ def main():
     states = plan[:k]
     log(f"[bench] timing {k} QP solve(s) {states} ({clamp_reason}); {since_start():.0f}s since start")
 
+    if args.profile_steps > 0:
+        # counter calibration for profiler runs: one kernel of known traffic,
+        # 1 GiB read and 1 GiB written (torch's elementwise negation)
+        cal = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        torch.neg(cal, out=torch.empty_like(cal))
+        torch.cuda.synchronize()
+        del cal
+    if args.reorth:
+        if args.mode != "single":
+            raise SystemExit("--reorth needs --mode single (the batched engine runs the plain recurrence)")
+        ctx.set_reorthogonalization(True)
     ctx.profile_enable(not args.graph)
     ctx.profile_reset()
     torch.cuda.synchronize()
@@ -670,7 +692,8 @@ def main():
                                ("served mode: the timed QP states are requests of one continuous-batching "
                                 "loop (all N_b bands of every state in each launch)" if args.mode == "served"
                                 else "single mode: one ngw_sigma_lanczos call per QP state") +
-                               (", hipGraph replay" if args.graph else ""),
+                               (", hipGraph replay" if args.graph else "") +
+                               (", full reorthogonalisation" if args.reorth else ""),
                    "qp_states": states, "parallelism": "single GPU", "tile_configs": tiles},
         "budget": {"wall_budget_s": args.budget_s, "steps_requested": args.steps, "steps_timed": k,
                    "clamp": clamp_reason, "warmup_lanczos_steps": w,

@@ -103,12 +103,35 @@ int ngw_set_reorthogonalization(ngw_context* ctx, int mode);
  * After it the operator entries (ngw_matvec_isdf_block,
  * ngw_polynomial_matvec_isdf_block, ngw_lanczos_spectra_isdf_block and the
  * _dev apply) compute the same y = R^2 x + 4 R^1/2 C^T M C R^1/2 x (TDA:
- * y = R x + 2 C^T M C x) with the three products as FP64 library GEMMs.
- * FP64 only; the Sigma_c / Sigma_x entries and ngw_autotune need the
- * factored operator and fail with NGW_ERR_STATE.  ngw_set_isdf switches back.
+ * y = R x + 2 C^T M C x) with the three products as this library's FP64
+ * GEMV / GEMM kernels.  FP64 only; the Sigma_c / Sigma_x entries and
+ * ngw_autotune need the factored operator and fail with NGW_ERR_STATE.
+ * ngw_set_isdf switches back.
  */
 int ngw_set_explicit_c(ngw_context* ctx, int n_intp, int ncv, const double* cmtrx, int ld_c,
                        const double* mmtrx, int ld_m, const double* sqrtr);
+
+/*
+ * Irrep-blocked operator of the lowcomsym routines, replaces the operator
+ * data of matvec_isdf_lowcomsym / polynomial_matvec_isdf_lowcomsym /
+ * lanczos_spectra_isdf_lowcomsym (shared/apply_matvec_lowcomsym.F90:31,
+ * :419; shared/lanczos_spectra_lowcomsym.F90:7): isdf_in%PsiV_intp_bl /
+ * PsiC_intp_bl gathered through lvrep / lcrep, Mmtrx_loc(:, :, ..., nmrep)
+ * and sqrtR.  Block k (k = 0 .. nblock-1) is one valence irrep and the
+ * conduction irrep whose product with it is nmrep (:158-161): its nv_blk[k]
+ * valence states are columns sum_{k'<k} nv_blk[k'] .. of psiv(ld_v, *), its
+ * nc_blk[k] conduction states the matching columns of psic(ld_c, *).  Pairs
+ * are numbered block after block, valence state outer and conduction state
+ * inner (the reference's cvstart order, :159-247), ncv = sum nv_blk nc_blk;
+ * sqrtr(ncv) in that order.  The operator entries then apply
+ * y = R^2 x + 4 R^1/2 C^T M C R^1/2 x (TDA: y = R x + 2 C^T M C x) in
+ * factored form, block by block, without forming C.  FP64 only; the
+ * Sigma_c / Sigma_x entries and ngw_autotune fail with NGW_ERR_STATE.
+ * ngw_set_isdf or ngw_set_explicit_c switches to another form.
+ */
+int ngw_set_isdf_blocks(ngw_context* ctx, int n_intp, int nblock, const int* nv_blk, const int* nc_blk,
+                        const double* psiv, int ld_v, const double* psic, int ld_c,
+                        const double* mmtrx, int ld_m, const double* sqrtr);
 
 /*
  * Operator apply on blksz vectors, replaces matvec_isdf_BLOCK

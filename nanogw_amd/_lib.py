@@ -31,6 +31,8 @@ _SIGS = {
     "ngw_set_reorthogonalization": ([_vp, C.c_int], C.c_int),
     "ngw_qp_newton_dev_noalloc": ([_vp, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
     "ngw_set_explicit_c": ([_vp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp], C.c_int),
+    "ngw_set_isdf_blocks": ([_vp, C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp],
+                            C.c_int),
     "ngw_matvec_isdf_block": ([_vp, _dp, _dp, C.c_int, C.c_int, C.c_int], C.c_int),
     "ngw_polynomial_matvec_isdf_block": ([_vp, _dp, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int],
                                          C.c_int),

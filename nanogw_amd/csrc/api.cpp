@@ -91,6 +91,7 @@ static void prof_drain(ngw_context* c)
 int operator_apply(ngw_context* ctx, const OperatorArgs& a)
 {
     if (ctx->explicit_c) return operator_explicit(ctx, a);
+    if (ctx->blocked) return operator_blocked(ctx, a);
     if (ctx->precision == PREC_F64) return operator_f64(ctx, a);
     return operator_lp2(ctx, a);
 }
@@ -109,6 +110,8 @@ static int check_factored(ngw_context* ctx)
     NGW_TRY(check_ctx(ctx));
     if (ctx->explicit_c)
         return fail(NGW_ERR_STATE, "operator given by an explicit C matrix: this entry needs ngw_set_isdf");
+    if (ctx->blocked)
+        return fail(NGW_ERR_STATE, "operator given by irrep blocks: this entry needs ngw_set_isdf");
     return NGW_OK;
 }
 
@@ -321,6 +324,8 @@ int ngw_set_precision(ngw_context* ctx, int precision)
     if (precision < NGW_PREC_F64 || precision > NGW_PREC_F16) return fail(NGW_ERR_ARG, "bad precision");
     if (ctx->explicit_c && precision != NGW_PREC_F64)
         return fail(NGW_ERR_STATE, "the explicit-C operator is FP64 only");
+    if (ctx->blocked && precision != NGW_PREC_F64)
+        return fail(NGW_ERR_STATE, "the irrep-blocked operator is FP64 only");
     ctx->precision = precision;
     if (precision != NGW_PREC_F64 && ctx->n_intp > 0) {
         NGW_HIP(hipSetDevice(ctx->device));

@@ -172,6 +172,9 @@ void explicit_release(ngw_context* ctx)
     ctx->ex_t.release();
     ctx->ex_s.release();
     ctx->explicit_c = false;
+    ctx->blocked = false;
+    ctx->blk_nv.clear();
+    ctx->blk_nc.clear();
 }
 
 }  // namespace ngw

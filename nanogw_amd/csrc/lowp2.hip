@@ -817,7 +817,9 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                 for (int r = 0; r < 4; ++r)
                     stg[hrow + 4 * (lane >> 4) + r][wn + 16 * jf + (lane & 15)] = acc.v[q][jf][r];
             __syncthreads();
-#pragma unroll 1
+            // two groups in flight where the tile runs at two waves per SIMD
+            // (its register budget has room), one at three waves per SIMD
+#pragma unroll(BM * BN <= 128 * 128 ? 1 : 2)
             for (int k = 0; k < PER; ++k) {
                 const int e = threadIdx.x + GNT * k;
                 const int rl = e / G, cg = e - rl * G;

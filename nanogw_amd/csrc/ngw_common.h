@@ -99,6 +99,11 @@ struct ngw_context {
     // one-vector products (scaled input, partial sums of split sums)
     bool explicit_c = false;
     ngw::DevBuf cexp, ex_x, ex_y, ex_t, ex_s;
+    // irrep-blocked pairs (ngw_set_isdf_blocks): psi holds the blocks'
+    // valence orbitals, then their conduction orbitals; block k pairs
+    // blk_nv[k] valence with blk_nc[k] conduction states
+    bool blocked = false;
+    std::vector<int> blk_nv, blk_nc;
     ngw::DevBuf gv_part;  // chunk sums of the one-vector GEMV (gemv_wide)
     // workspaces
     ngw::DevBuf tpart, ubuf, w0, w1, w2, w3, w4, ab, zz, spec, misc, misc2, staging;

@@ -129,9 +129,12 @@ int gemv_wide(hipStream_t s, const double* A, long lda, int m, int n, const doub
 // few vectors; partials in `part`, added in range order.
 int gemm_f64_split(hipStream_t s, int m, int n, int k, const double* a, long lda, const double* b, long ldb,
                    double* c, long ldc, DevBuf& part);
-// The operator with an explicit pair matrix C (rocBLAS products, FP64).
+// The operator with an explicit pair matrix C (FP64).
 int operator_explicit(ngw_context* ctx, const OperatorArgs& a);
+// Releases the explicit-C form and leaves the irrep-blocked form.
 void explicit_release(ngw_context* ctx);
+// The irrep-blocked operator of the lowcomsym routines in factored form (FP64).
+int operator_blocked(ngw_context* ctx, const OperatorArgs& a);
 
 // Dispatch on the operator form and ctx->precision.
 int operator_apply(ngw_context* ctx, const OperatorArgs& a);

@@ -59,6 +59,15 @@ module ngw_c
       real(c_double), intent(in) :: cmtrx(*), mmtrx(*), sqrtr(*)
     end function ngw_set_explicit_c
 
+    integer(c_int) function ngw_set_isdf_blocks(ctx, n_intp, nblock, nv_blk, nc_blk, psiv, ld_v, psic, ld_c, &
+                                                mmtrx, ld_m, sqrtr) bind(C, name='ngw_set_isdf_blocks')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_intp, nblock, ld_v, ld_c, ld_m
+      integer(c_int), intent(in) :: nv_blk(*), nc_blk(*)
+      real(c_double), intent(in) :: psiv(*), psic(*), mmtrx(*), sqrtr(*)
+    end function ngw_set_isdf_blocks
+
     integer(c_int) function ngw_matvec_isdf_block(ctx, vec, hvec, ncv, blksz, tamm_d) &
       bind(C, name='ngw_matvec_isdf_block')
       import :: c_ptr, c_int, c_double

@@ -22,8 +22,9 @@
 ! operator, same scope.
 !
 ! lowcomsym routines (CPU-build argument lists): the irrep-blocked operator
-! uploaded as an explicit C over the pairs of representation nmrep, built
-! here in the reference's pair order (see ngw_lowcomsym_operator).
+! over the pairs of representation nmrep, uploaded as its orbital blocks and
+! applied in factored form in the reference's pair order (see
+! ngw_lowcomsym_operator).
 !
 ! Single-vector routines (matvec_isdf, polynomial_matvec_isdf,
 ! lanczos_spectra_isdf): the caller's tmpCmtrx, Mmtrx and sqrtR are uploaded
@@ -62,11 +63,11 @@ module ngw_session
   type(c_ptr), save :: up_c = c_null_ptr, up_mx = c_null_ptr
   integer, save :: up_cnintp = -1, up_cncv = -1
   real(c_double), allocatable, save :: up_sqrtr(:), up_csamp(:), up_msamp(:)
-  ! irrep-blocked (lowcomsym) explicit operator: keyed by its Psi blocks,
-  ! Mmtrx_loc, nmrep and samples
-  type(c_ptr), save :: up_lv = c_null_ptr, up_lcp = c_null_ptr, up_lm = c_null_ptr
-  integer, save :: up_lnmrep = -1, up_lncv = -1, up_lnintp = -1
-  real(c_double), allocatable, save :: up_lsqrtr(:), up_lsamp(:)
+  ! irrep-blocked (lowcomsym) operator: full copies of what was uploaded
+  ! (the gathered Psi blocks, the block sizes, Mmtrx_loc of nmrep, sqrtR)
+  integer, save :: up_lncv = -1
+  integer(c_int), allocatable, save :: up_lnvb(:), up_lncb(:)
+  real(c_double), allocatable, save :: up_lsqrtr(:), up_lpv(:, :), up_lpc(:, :), up_lmm(:, :)
 
 contains
 
@@ -240,57 +241,39 @@ contains
     deallocate (a_elmt, b_elmt)
   end subroutine ngw_lanczos_single
 
-  ! The irrep-blocked operator of matvec_isdf_lowcomsym, uploaded in explicit
-  ! form unless the same one is resident.  cm(n_intp, ncv) holds the pair
-  ! products in the reference's order; key_* identify the caller's arrays
-  ! (Psi_v and Psi_c blocks, Mmtrx_loc) and samp a sample of their values.
-  logical function ngw_lowcomsym_resident(kv, kc, km, nmrep, n_intp, ncv, sqrtr, samp)
-    type(c_ptr), intent(in) :: kv, kc, km
-    integer, intent(in) :: nmrep, n_intp, ncv
-    real(c_double), intent(in) :: sqrtr(ncv), samp(:)
-    ngw_lowcomsym_resident = .false.
-    if (.not. c_associated(ngw_ctx)) return
-    if (up_lncv /= ncv .or. up_lnintp /= n_intp .or. up_lnmrep /= nmrep) return
-    if (.not. (c_associated(up_lv, kv) .and. c_associated(up_lcp, kc) .and. c_associated(up_lm, km))) return
-    if (size(up_lsamp) /= size(samp)) return
-    ngw_lowcomsym_resident = all(up_lsqrtr == sqrtr) .and. all(up_lsamp == samp)
-  end function ngw_lowcomsym_resident
-
-  subroutine ngw_upload_lowcomsym(kv, kc, km, nmrep, n_intp, ncv, cm, ld_m, mmtrx, sqrtr, samp)
-    type(c_ptr), intent(in) :: kv, kc, km
-    integer, intent(in) :: nmrep, n_intp, ncv, ld_m
-    real(c_double), intent(in) :: cm(n_intp, ncv), mmtrx(ld_m, n_intp), sqrtr(ncv), samp(:)
+  ! The irrep-blocked operator of matvec_isdf_lowcomsym in factored form:
+  ! pv / pc hold the blocks' valence / conduction orbitals at the ISDF
+  ! points, nvb / ncb the block sizes.  Uploaded unless the resident copy is
+  ! equal in every value (in-place changes of the caller's arrays are seen).
+  subroutine ngw_ensure_lowcomsym(n_intp, nblk, nvb, ncb, pv, pc, ld_m, mmtrx, ncv, sqrtr)
+    integer, intent(in) :: n_intp, nblk, ld_m, ncv
+    integer(c_int), intent(in) :: nvb(nblk), ncb(nblk)
+    real(c_double), intent(in) :: pv(:, :), pc(:, :), mmtrx(ld_m, n_intp), sqrtr(ncv)
     call ngw_start()
-    call ngw_check(ngw_set_explicit_c(ngw_ctx, int(n_intp, c_int), int(ncv, c_int), cm, int(n_intp, c_int), &
-                                      mmtrx, int(ld_m, c_int), sqrtr), 'ngw_set_explicit_c (lowcomsym)')
-    up_lv = kv
-    up_lcp = kc
-    up_lm = km
-    up_lnmrep = nmrep
+    if (up_lncv == ncv .and. allocated(up_lnvb)) then
+      if (size(up_lnvb) == nblk .and. size(up_lpv, 1) == n_intp .and. size(up_lpv, 2) == size(pv, 2) .and. &
+          size(up_lpc, 2) == size(pc, 2)) then
+        if (all(up_lnvb == nvb) .and. all(up_lncb == ncb) .and. all(up_lsqrtr == sqrtr) .and. &
+            all(up_lpv == pv) .and. all(up_lpc == pc) .and. all(up_lmm == mmtrx(1:n_intp, 1:n_intp))) return
+      end if
+    end if
+    call ngw_check(ngw_set_isdf_blocks(ngw_ctx, int(n_intp, c_int), int(nblk, c_int), nvb, ncb, pv, &
+                                       int(n_intp, c_int), pc, int(n_intp, c_int), mmtrx, int(ld_m, c_int), sqrtr), &
+                   'ngw_set_isdf_blocks (lowcomsym)')
     up_lncv = ncv
-    up_lnintp = n_intp
+    up_lnvb = nvb
+    up_lncb = ncb
     up_lsqrtr = sqrtr
-    up_lsamp = samp
+    up_lpv = pv
+    up_lpc = pc
+    up_lmm = mmtrx(1:n_intp, 1:n_intp)
     ! the other operator forms are gone from the context
     up_psi = c_null_ptr
     up_m = c_null_ptr
     up_nintp = -1
     up_c = c_null_ptr
     up_cnintp = -1
-  end subroutine ngw_upload_lowcomsym
-
-  ! Largest explicit pair matrix the lowcomsym routines form (bytes): 2 GiB,
-  ! or NGW_LOWCOMSYM_MAX_GIB from the environment.
-  real(c_double) function ngw_lowcomsym_limit()
-    character(len=32) :: val
-    integer :: st, ios
-    real(c_double) :: gib
-    ngw_lowcomsym_limit = 2.0d0 * 1024.0d0**3
-    call get_environment_variable('NGW_LOWCOMSYM_MAX_GIB', val, status=st)
-    if (st /= 0) return
-    read (val, *, iostat=ios) gib
-    if (ios == 0 .and. gib > 0.0d0) ngw_lowcomsym_limit = gib * 1024.0d0**3
-  end function ngw_lowcomsym_limit
+  end subroutine ngw_ensure_lowcomsym
 
   subroutine ngw_release()
     integer(c_int) :: st
@@ -515,12 +498,14 @@ subroutine lanczos_spectra_isdf_UltraLowMem(v0, v0_norm, ncv_loc, zz, nz, niter,
                           'lanczos_spectra_isdf_UltraLowMem')
 end subroutine lanczos_spectra_isdf_UltraLowMem
 
-! Explicit pair matrix of the irrep-blocked operator in the pair order of
-! matvec_isdf_lowcomsym (shared/apply_matvec_lowcomsym.F90:155-247): for each
+! The irrep-blocked operator of matvec_isdf_lowcomsym
+! (shared/apply_matvec_lowcomsym.F90:155-412) in factored form: for each
 ! valence irrep jvrep the conduction irrep jcrep with prod(jcrep, jvrep) =
-! nmrep; valence states of jvrep outer, conduction states of jcrep inner;
-! column = PsiC_intp_bl(:, lcrep(ic)) * PsiV_intp_bl(:, lvrep(iv)).  The
-! operator (a single rank, w_grp%npes = 1) is uploaded unless resident.
+! nmrep is one block; its orbitals are gathered through lvrep / lcrep
+! (PsiV_intp_bl(:, lvrep(iv)), PsiC_intp_bl(:, lcrep(ic))), and pairs run
+! block after block, valence outer, conduction inner (the reference's
+! cvstart order, :159-247).  The pair matrix is not formed.  A single rank
+! (w_grp%npes = 1).
 subroutine ngw_lowcomsym_operator(ncv_loc, isdf_in, sqrtR, nmrep, gvec)
   NGW_TYPES
   use ngw_session
@@ -529,53 +514,45 @@ subroutine ngw_lowcomsym_operator(ncv_loc, isdf_in, sqrtR, nmrep, gvec)
   type(ISDF), intent(in) :: isdf_in
   real(dp), intent(in) :: sqrtR(ncv_loc)
   type(gspace), intent(in) :: gvec
-  real(dp), allocatable :: cm(:, :), samp(:)
-  integer :: n, jvrep, jcrep, iv, ic, icv, k, nv_bl, nc_bl
-  type(c_ptr) :: kv, kc, km
+  real(dp), allocatable :: pv(:, :), pc(:, :)
+  integer(c_int), allocatable :: nvb(:), ncb(:), jvs(:), jcs(:)
+  integer :: n, ntr, nblk, jvrep, jcrep, iv, ic, kv, kc, k
+  integer(8) :: npair
   if (w_grp%npes /= 1) stop 'libngw drop-in: one rank per GPU (w_grp%npes = 1)'
   n = isdf_in%n_intp_r
-  nv_bl = size(isdf_in%PsiV_intp_bl, 2)
-  nc_bl = size(isdf_in%PsiC_intp_bl, 2)
-  kv = c_loc(isdf_in%PsiV_intp_bl(1, 1, 1, 1))
-  kc = c_loc(isdf_in%PsiC_intp_bl(1, 1, 1, 1))
-  km = c_loc(isdf_in%Mmtrx_loc(1, 1, 1, 1, 1, 1, nmrep))
-  allocate (samp(0))
-  do k = 1, min(n * nv_bl, 1024)
-    samp = [samp, isdf_in%PsiV_intp_bl(1 + mod((k - 1) * 7919, n), 1 + mod((k - 1) * 104729, nv_bl), 1, 1)]
-  end do
-  do k = 1, min(n * nc_bl, 1024)
-    samp = [samp, isdf_in%PsiC_intp_bl(1 + mod((k - 1) * 7919, n), 1 + mod((k - 1) * 104729, nc_bl), 1, 1)]
-  end do
-  do k = 1, min(n, 1024)
-    samp = [samp, isdf_in%Mmtrx_loc(k, 1 + mod((k - 1) * 7919, n), 1, 1, 1, 1, nmrep)]
-  end do
-  if (ngw_lowcomsym_resident(kv, kc, km, nmrep, n, ncv_loc, sqrtR, samp)) return
-  if (8.0d0 * real(n, 8) * real(ncv_loc, 8) > ngw_lowcomsym_limit()) then
-    write (6, '(a, f8.2, a, f8.2, a)') ' libngw drop-in: the lowcomsym pair matrix would take ', &
-      8.0d0 * real(n, 8) * real(ncv_loc, 8) / 1024.0d0**3, ' GiB on the host and the device, above the ', &
-      ngw_lowcomsym_limit() / 1024.0d0**3, ' GiB limit (environment NGW_LOWCOMSYM_MAX_GIB)'
-    stop 1
-  end if
-  allocate (cm(n, ncv_loc))
-  icv = 0
-  do jvrep = 1, gvec%syms%ntrans
-    do jcrep = 1, gvec%syms%ntrans
+  ntr = gvec%syms%ntrans
+  allocate (nvb(ntr), ncb(ntr), jvs(ntr), jcs(ntr))
+  nblk = 0
+  npair = 0
+  do jvrep = 1, ntr
+    do jcrep = 1, ntr
       if (gvec%syms%prod(jcrep, jvrep) == nmrep) exit
     end do
-    if (jcrep > gvec%syms%ntrans) cycle
-    do iv = isdf_in%lvrep_bound(jvrep, 1), isdf_in%lvrep_bound(jvrep, 2)
-      do ic = isdf_in%lcrep_bound(jcrep, 1), isdf_in%lcrep_bound(jcrep, 2)
-        icv = icv + 1
-        if (icv > ncv_loc) stop 'libngw drop-in: lowcomsym pair count exceeds ncv_loc'
-        cm(:, icv) = isdf_in%PsiC_intp_bl(1:n, isdf_in%lcrep(ic), 1, 1) * &
-                     isdf_in%PsiV_intp_bl(1:n, isdf_in%lvrep(iv), 1, 1)
-      end do
+    if (jcrep > ntr) cycle
+    nblk = nblk + 1
+    jvs(nblk) = jvrep
+    jcs(nblk) = jcrep
+    nvb(nblk) = max(0, isdf_in%lvrep_bound(jvrep, 2) - isdf_in%lvrep_bound(jvrep, 1) + 1)
+    ncb(nblk) = max(0, isdf_in%lcrep_bound(jcrep, 2) - isdf_in%lcrep_bound(jcrep, 1) + 1)
+    npair = npair + int(nvb(nblk), 8) * int(ncb(nblk), 8)
+  end do
+  if (npair /= int(ncv_loc, 8)) stop 'libngw drop-in: lowcomsym pair count differs from ncv_loc'
+  allocate (pv(n, sum(nvb(1:nblk))), pc(n, sum(ncb(1:nblk))))
+  kv = 0
+  kc = 0
+  do k = 1, nblk
+    do iv = isdf_in%lvrep_bound(jvs(k), 1), isdf_in%lvrep_bound(jvs(k), 2)
+      kv = kv + 1
+      pv(:, kv) = isdf_in%PsiV_intp_bl(1:n, isdf_in%lvrep(iv), 1, 1)
+    end do
+    do ic = isdf_in%lcrep_bound(jcs(k), 1), isdf_in%lcrep_bound(jcs(k), 2)
+      kc = kc + 1
+      pc(:, kc) = isdf_in%PsiC_intp_bl(1:n, isdf_in%lcrep(ic), 1, 1)
     end do
   end do
-  if (icv /= ncv_loc) stop 'libngw drop-in: lowcomsym pair count differs from ncv_loc'
-  call ngw_upload_lowcomsym(kv, kc, km, nmrep, n, ncv_loc, cm, size(isdf_in%Mmtrx_loc, 1), &
-                            isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, nmrep), sqrtR, samp)
-  deallocate (cm, samp)
+  call ngw_ensure_lowcomsym(n, nblk, nvb(1:nblk), ncb(1:nblk), pv, pc, size(isdf_in%Mmtrx_loc, 1), &
+                            isdf_in%Mmtrx_loc(:, :, 1, 1, 1, 1, nmrep), ncv_loc, sqrtR)
+  deallocate (pv, pc, nvb, ncb, jvs, jcs)
 end subroutine ngw_lowcomsym_operator
 
 ! shared/apply_matvec_lowcomsym.F90:31 (CPU build's argument list)

@@ -9,7 +9,9 @@
 ! stream file (argument 2).  Between the BLOCK calls and the single-vector
 ! ones the conduction energies in wfn%e1 are raised by 0.1 Ry in place for
 ! one more matvec_isdf_BLOCK call (the update an SCGW iteration makes), then
-! restored: the drop-in must rebuild its resident operator both times.
+! restored: the drop-in must rebuild its resident operator both times.  After
+! the lowcomsym calls one value of PsiC_intp_bl is changed in place for one
+! more matvec_isdf_lowcomsym call.
 program ngw_dropin_check
   use ngw_dropin_selftest_types
   use ngw_session, only: ngw_release
@@ -25,7 +27,7 @@ program ngw_dropin_check
   integer, allocatable :: cvpair2(:, :)
   type(gspace) :: gvec
   integer :: nv1, nc1, nsub, jr, k
-  real(dp), allocatable :: sqrtrs(:), vs(:, :), hs(:, :), hst(:, :), ps(:, :), vsn(:)
+  real(dp), allocatable :: sqrtrs(:), vs(:, :), hs(:, :), hst(:, :), ps(:, :), hs2(:, :), vsn(:)
   complex(dp), allocatable :: specs(:, :), spect(:, :), zzs(:, :)
   real(dp), allocatable :: hu(:), pu(:), veig2(:)
   complex(dp), allocatable :: specu(:), speigu(:)
@@ -140,8 +142,12 @@ program ngw_dropin_check
                                       2, 1, gvec)
   call lanczos_spectra_isdf_lowcomsym(vs, vsn, nsub, zzs, nz, niter, isdf_in, sqrtrs, poly, npoly, spect, .true., &
                                       2, 1, gvec)
+  ! one orbital value changed in place: the drop-in sees it and rebuilds
+  allocate (hs2(nsub, 2))
+  isdf_in%PsiC_intp_bl(3, 2, 1, 1) = isdf_in%PsiC_intp_bl(3, 2, 1, 1) + 0.5_dp
+  call matvec_isdf_lowcomsym(vs, hs2, nsub, isdf_in, sqrtrs, .false., 2, 1, gvec)
   call ngw_release()
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u) hvec, spectra, h1, p1, spec1, speig, hu, pu, specu, speigu, hs, hst, ps, specs, spect, hvec_e
+  write (u) hvec, spectra, h1, p1, spec1, speig, hu, pu, specu, speigu, hs, hst, ps, specs, spect, hs2, hvec_e
   close (u)
 end program ngw_dropin_check

@@ -107,6 +107,30 @@ class IsdfContext:
         self.nval = self.ncond = self.nband = 0
         self.e1 = None
 
+    def set_isdf_blocks(self, psiv, psic, nv_blk, nc_blk, mmtrx, sqrt_r):
+        """The irrep-blocked operator of matvec_isdf_lowcomsym
+        (shared/apply_matvec_lowcomsym.F90:155-412) in factored form: psiv
+        (n_intp, sum nv_blk) and psic (n_intp, sum nc_blk) hold the blocks'
+        orbitals block after block; pairs are numbered block after block,
+        valence outer, conduction inner; sqrt_r (ncv,) in that order."""
+        psiv, psic, mmtrx, sqrt_r = _f64(psiv), _f64(psic), _f64(mmtrx), _f64(sqrt_r)
+        nv_blk = np.ascontiguousarray(nv_blk, dtype=np.int32)
+        nc_blk = np.ascontiguousarray(nc_blk, dtype=np.int32)
+        n_intp = psiv.shape[0]
+        ncv = int(np.dot(nv_blk.astype(np.int64), nc_blk.astype(np.int64)))
+        if (nv_blk.shape != nc_blk.shape or psiv.shape != (n_intp, int(nv_blk.sum()))
+                or psic.shape != (n_intp, int(nc_blk.sum())) or mmtrx.shape != (n_intp, n_intp)
+                or sqrt_r.shape != (ncv,)):
+            raise ValueError("psiv (n_intp, sum nv_blk), psic (n_intp, sum nc_blk), Mmtrx (n_intp, n_intp), "
+                             "sqrtR (sum nv_blk nc_blk,)")
+        ip = C.POINTER(C.c_int)
+        _lib.check(self._lib.ngw_set_isdf_blocks(self._h, n_intp, len(nv_blk), nv_blk.ctypes.data_as(ip),
+                                                 nc_blk.ctypes.data_as(ip), _p(psiv), n_intp, _p(psic), n_intp,
+                                                 _p(mmtrx), n_intp, _p(sqrt_r)), "ngw_set_isdf_blocks")
+        self.n_intp, self._ncv = n_intp, ncv
+        self.nval = self.ncond = self.nband = 0
+        self.e1 = None
+
     def set_precision(self, precision):
         p = PREC[precision] if isinstance(precision, str) else int(precision)
         _lib.check(self._lib.ngw_set_precision(self._h, p), "ngw_set_precision")

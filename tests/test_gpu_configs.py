@@ -22,9 +22,13 @@ Nmu = 12 nv, 16 QP states each; reduced niter):
   * three molecules of the seeded batch spanning nv (smallest, median, and
     nv = 60) in one FP16 batch with hipGraph replay: each result bitwise
     equal to the same request run alone;
-  * QP energies against the FP64 batch: BF16x3 within 1 meV for all 16
-    states, FP16 within 1 meV for HOMO-1 .. LUMO+1 (deep valence states are
-    printed: FP16 moves them by up to tens of meV);
+  * QP energies against the FP64 batch (nv = 9 and 38; nv = 60 is
+    rounding-chaotic at 16 steps in every precision): BF16x3 within 1 meV for
+    all 16 states; FP16 printed (it moves the valence states of the nv = 38
+    molecule by up to tens of meV);
+  * on the nv = 38 molecule, full reorthogonalisation brings FP16 within
+    1 meV of FP64 on the same 16 states, so the FP16 valence error is the
+    recurrence's loss of orthogonality driven by the operator's rounding;
   * the FP64 batch equal to the single-context path, and the single-context
     path equal to the oracle on a band sample (8 Green's-function bands),
     within 1e-8 or 3x the oracle's own rounding envelope (oracle runs on
@@ -190,11 +194,11 @@ def _run_alone(b, sid, qp, nen, deltae, ecuts, niter, poly, use_graph=True):
 
 def test_batched_f16_alone_equals_mixed_and_qp_vs_f64(ngw_ctx, batch_mols):
     """FP16 batch: bitwise equal alone and in the mix.  QP energies against
-    the FP64 batch: BF16x3 within 1 meV for all 16 states; FP16 within 1 meV
-    for the four states around the gap (HOMO-1 .. LUMO+1).  Deeper valence
-    states are more sensitive to the operator's rounding in FP16 (measured up
-    to 39 meV on the nv = 38 molecule at 16 steps, where BF16x3 stays below
-    0.03 meV), so they are printed, not bounded."""
+    the FP64 batch (nv = 9 and 38): BF16x3 within 1 meV for all 16 states.  FP16 is printed,
+    not bounded: on the nv = 38 molecule at 16 steps it moves the valence
+    states by 6-39 meV (HOMO 23 meV) while the conduction states stay within
+    0.23 meV and BF16x3 within 0.03 meV everywhere; the batched and served
+    workloads therefore state their QP accuracy for BF16x3."""
     from nanogw_amd import synth
     from nanogw_amd.batch import GwBatch
     from nanogw_amd.lanczos import lanczos_polynomial
@@ -225,19 +229,21 @@ def test_batched_f16_alone_equals_mixed_and_qp_vs_f64(ngw_ctx, batch_mols):
         finally:
             b.close()
             ngw_ctx.set_precision("f64")
-    for k, m in enumerate(mols):
+    # QP energies of nv = 9 and nv = 38 only: the nv = 60 molecule's
+    # recurrence is rounding-chaotic past 8 steps already in FP64 (its
+    # perturbation envelope is 7e-5 at 8 steps, 4e-2 at 12), so its
+    # precisions cannot be compared at 16 steps
+    for k, m in enumerate(mols[:2]):
         e = m.e1[qps[k]] * RYD
         sx = synth.exchange_diag(m.psi_intp, m.mmtrx, m.nv, qps[k]) * RYD
         q = {p: ngw_ctx.qp_newton(sig[p][k] * RYD, deltae * RYD, e, e, sx, np.zeros(len(e))) for p in sig}
         assert all(not np.any(q[p]["ierr"]) for p in q)
         d16 = 1000.0 * (q["f16"]["eqp"] - q["f64"]["eqp"])
         d3 = 1000.0 * (q["bf16x3"]["eqp"] - q["f64"]["eqp"])
-        gap = np.isin(qps[k], [m.nv - 2, m.nv - 1, m.nv, m.nv + 1])
         print(f"batched nv={m.nv}, {NITER_BATCH} steps, E_qp - E_qp(FP64) in meV per state {qps[k][0]}..{qps[k][-1]}:")
         print("  FP16   " + " ".join(f"{v:8.3f}" for v in d16))
         print("  BF16x3 " + " ".join(f"{v:8.3f}" for v in d3))
         assert np.abs(d3).max() < 1.0
-        assert np.abs(d16[gap]).max() < 1.0
 
 
 def test_batched_f64_vs_single_and_oracle_band_sample(ngw_ctx, batch_mols):
@@ -308,6 +314,32 @@ def test_reorth_stabilises_batched_molecule(ngw_ctx, batch_mols):
     assert all(not np.any(res[k]["ierr"]) for k in ("r64", "r64p", "r16"))
     assert d("r64", "r64p") < 1e-6
     assert d("r16", "r64") < 1.0
+
+    # The batched workload's 16-state window at 16 steps: FP64's recurrence is
+    # still stable there (perturbation envelope 5e-11), FP16's operator
+    # rounding is not absorbed by it.  Printed: FP16 plain and reorthogonalised
+    # against plain FP64; bounded: FP16 reorthogonalised within 1 meV.
+    qw = synth.qp_window(m.nv, m.nb, 16)
+    e = m.e1[qw] * RYD
+    sx = synth.exchange_diag(m.psi_intp, m.mmtrx, m.nv, qw) * RYD
+    win = {}
+    for key, prec, reorth in (("p64", "f64", False), ("p16", "f16", False), ("r16", "f16", True),
+                              ("r64", "f64", True)):
+        ngw_ctx.set_isdf(m.psi_intp, m.mmtrx, m.e1, m.nv)
+        ngw_ctx.set_precision(prec)
+        ngw_ctx.set_tile_config(0, 0)
+        ngw_ctx.set_reorthogonalization(reorth)
+        try:
+            sig = ngw_ctx.calculate_sigma_lanczos(qw, nen, deltae, ecuts, 16, poly)
+        finally:
+            ngw_ctx.set_reorthogonalization(False)
+            ngw_ctx.set_precision("f64")
+        win[key] = ngw_ctx.qp_newton(sig * RYD, deltae * RYD, e, e, sx, np.zeros(len(e)))
+        assert not np.any(win[key]["ierr"])
+    for key in ("p16", "r16", "r64"):
+        dv = 1000.0 * (win[key]["eqp"] - win["p64"]["eqp"])
+        print(f"  16 steps, states {qw[0]}..{qw[-1]}, {key} - plain FP64 (meV): " + " ".join(f"{v:.3f}" for v in dv))
+    assert 1000.0 * np.abs(win["r16"]["eqp"] - win["r64"]["eqp"]).max() < 1.0
 
 
 # ---------------------------------------------------------------- served stream

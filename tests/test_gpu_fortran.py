@@ -25,29 +25,6 @@ def _write_inputs(fin, m, blksz, niter, poly, vec, zz, veig):
         veig.tofile(f)
 
 
-def test_fortran_dropin_lowcomsym_size_limit(gpu, mini, tmp_path):
-    """The lowcomsym routines form the irrep's pair matrix on the host; above
-    the stated limit (NGW_LOWCOMSYM_MAX_GIB, default 2 GiB) they stop with a
-    message before allocating it.  Here the limit is set below the check
-    program's tiny pair matrix."""
-    from nanogw_amd.build import DROPIN_CHECK
-    from nanogw_amd.lanczos import lanczos_polynomial
-
-    if not os.path.exists(DROPIN_CHECK):
-        pytest.skip("Fortran drop-in check program not built (no amdflang)")
-    m = mini
-    ncv, blksz, niter = m.nv * m.nc, 3, 20
-    poly = lanczos_polynomial(m.e1, m.nv, 12)
-    rng = np.random.default_rng(11)
-    zz = rng.uniform(-1, 3, size=(17, blksz)) + 0.03j
-    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    _write_inputs(fin, m, blksz, niter, poly, rng.normal(size=(ncv, blksz)), zz, rng.normal(size=ncv))
-    env = dict(os.environ, NGW_LOWCOMSYM_MAX_GIB="1e-9")
-    r = subprocess.run([DROPIN_CHECK, str(fin), str(fout)], capture_output=True, text=True, timeout=120, env=env)
-    assert r.returncode != 0
-    assert "lowcomsym pair matrix would take" in r.stdout + r.stderr
-
-
 def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     from nanogw_amd.build import DROPIN_CHECK
     from nanogw_amd.lanczos import lanczos_polynomial
@@ -96,7 +73,8 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
     pairs += [(v, cl[c]) for v in range(nv1, nv) for c in range(nc1, nc)]
     nsub = len(pairs)
     hvec_e = raw[-ncv * blksz:].reshape(blksz, ncv).T
-    blk = raw[o4:-ncv * blksz].reshape(-1)
+    blk = raw[o4:-ncv * blksz - 2 * nsub].reshape(-1)
+    hs2 = raw[-ncv * blksz - 2 * nsub:-ncv * blksz].reshape(2, nsub).T
     hs = blk[:2 * nsub].reshape(2, nsub).T
     hst = blk[2 * nsub:4 * nsub].reshape(2, nsub).T
     ps = blk[4 * nsub:6 * nsub].reshape(2, nsub).T
@@ -146,6 +124,15 @@ def test_fortran_dropin_matches_oracle(gpu, mini, tmp_path):
 
     assert rel_err(hs, sub_apply(xs, False)) < 1e-11
     assert rel_err(hst, sub_apply(xs, True)) < 1e-11
+    # PsiC_intp_bl(3, 2) changed in place (conduction state 2, 1-based, is
+    # band nv + 1 of Psi): the drop-in rebuilt the blocked operator
+    psi2 = m.psi_intp.copy()
+    psi2[2, nv + 1] += 0.5
+    xf = np.zeros((ncv, 2))
+    xf[idx] = xs
+    ref2 = O.matvec_block(psi2, m.mmtrx, s, xf, nv, nc)[idx]
+    print("lowcomsym after in-place change", rel_err(hs2, ref2), rel_err(hs2, hs))
+    assert rel_err(hs2, ref2) < 1e-11 and rel_err(hs2, hs) > 1e-6
     cms = np.asfortranarray(O.pair_cmtrx(m.psi_intp, nv, nc)[:, idx])
     for k in range(2):
         assert rel_err(ps[:, k], O.polynomial_matvec_explicit(cms, m.mmtrx, s[idx], xs[:, k], poly)) < 1e-10
