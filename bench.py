@@ -273,7 +273,7 @@ def run_batched(args):
                                         keep_zeta=False, ridge=synth.BENCH_RIDGE))
         if i % 8 == 7 or i == len(specs) - 1:
             log(f"[bench] c4: {i + 1}/{len(specs)} molecules built ({time.time() - t:.1f}s)")
-    niter = specs[0]["niter"]
+    niter = args.profile_steps if args.profile_steps > 0 else specs[0]["niter"]
     sd = synth.sigma_defaults(8)
     nen, deltae, ecuts = sd["nen"], sd["deltae"], sd["ecuts"]
     # tile configurations tuned on the median molecule

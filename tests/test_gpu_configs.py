@@ -318,7 +318,10 @@ def test_reorth_stabilises_batched_molecule(ngw_ctx, batch_mols):
     # The batched workload's 16-state window at 16 steps: FP64's recurrence is
     # still stable there (perturbation envelope 5e-11), FP16's operator
     # rounding is not absorbed by it.  Printed: FP16 plain and reorthogonalised
-    # against plain FP64; bounded: FP16 reorthogonalised within 1 meV.
+    # against plain FP64 (measured: plain FP16 moves the eight valence states
+    # by 6.8-38.9 meV, reorthogonalised FP16 stays within 0.13 meV, and
+    # reorthogonalised FP64 equals plain FP64 to 1e-3 meV); bounded: FP16
+    # reorthogonalised within 1 meV.
     qw = synth.qp_window(m.nv, m.nb, 16)
     e = m.e1[qw] * RYD
     sx = synth.exchange_diag(m.psi_intp, m.mmtrx, m.nv, qw) * RYD
