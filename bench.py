@@ -482,6 +482,8 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="served mode: replay each Lanczos step as a hipGraph (no per-kernel timers)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tiles", default="",
+                    help="K1,K3 tile configuration indices instead of the autotuner's choice (profiler runs)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile configurations instead of timing them on this shape")
     ap.add_argument("--fp64-steps", type=int, default=3,
@@ -549,7 +551,11 @@ def main():
     ctx.set_precision(args.precision)
 
     tiles = None
-    if args.precision != "f64" and not args.no_autotune:
+    if args.precision != "f64" and args.tiles:
+        tiles = [int(v) for v in args.tiles.split(",")]
+        ctx.set_tile_config(tiles[0], tiles[1])
+        log(f"[bench] tile configurations given: K1 config {tiles[0]}, K3 config {tiles[1]}")
+    elif args.precision != "f64" and not args.no_autotune:
         t = time.time()
         tiles = ctx.autotune(nb, reps=2)
         log(f"[bench] autotune on {nb} vectors: K1 config {tiles[0]}, K3 config {tiles[1]} "

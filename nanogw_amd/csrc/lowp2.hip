@@ -819,7 +819,8 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
             __syncthreads();
             // two groups in flight where the tile runs at two waves per SIMD
             // (its register budget has room), one at three waves per SIMD
-#pragma unroll(BM * BN <= 128 * 128 ? 1 : 2)
+            constexpr int GUNROLL = BM * BN <= 128 * 128 ? 1 : 2;
+#pragma unroll GUNROLL
             for (int k = 0; k < PER; ++k) {
                 const int e = threadIdx.x + GNT * k;
                 const int rl = e / G, cg = e - rl * G;

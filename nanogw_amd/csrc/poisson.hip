@@ -6,8 +6,11 @@
 // of Zcreate_coul_0D (:628-691), C2R FFT, pick the grid points and scale by
 // 1/N.  Columns are processed in batches: one hand-written scatter kernel,
 // one batched hipFFT D2Z, one kernel-multiply kernel, one batched Z2D, one
-// gather kernel; then M(:, batch) = zeta^T (V zeta_batch) on the FP64 MFMA
-// GEMM (isdf_parallel_sym_UltraLowMem.F90:865-867).
+// gather kernel.  V zeta is kept for a group of batches (all columns when
+// they fit in 16 GiB) and M(:, group) = zeta^T (V zeta_group) is one FP64
+// MFMA GEMM per group (isdf_parallel_sym_UltraLowMem.F90:865-867): a GEMM
+// per FFT batch (33 columns at 158^3) would give too few tiles to fill the
+// GPU.
 
 #include <hipfft/hipfft.h>
 
@@ -130,13 +133,16 @@ static int coulomb_impl(ngw_context* ctx, int ngrid, int n_intp, const double* z
     // batch so that the real and complex boxes stay near 2 GiB
     int nb = (int)std::max(1L, std::min((long)n_intp, (2L << 30) / (8 * boxsz + 16 * csz)));
     nb = std::min(nb, 256);
+    // columns of V zeta kept for one GEMM: a multiple of nb, up to 16 GiB
+    const long vz_cols = std::max(1L, (16L << 30) / (8L * ngrid));
+    const int grp = (int)std::min((long)n_intp, std::max((long)nb, vz_cols / nb * nb));
     DevBuf dzeta, dvz, dbox, dfbox, dvc, doff, dgrid, dm;
     struct Guard {
         DevBuf* b[8];
         ~Guard() { for (DevBuf* x : b) x->release(); }
     } guard{{&dzeta, &dvz, &dbox, &dfbox, &dvc, &doff, &dgrid, &dm}};
     if (!zeta_dev) NGW_TRY(dzeta.ensure(sizeof(double) * (size_t)ngrid * n_intp));
-    NGW_TRY(dvz.ensure(sizeof(double) * (size_t)ngrid * nb));
+    NGW_TRY(dvz.ensure(sizeof(double) * (size_t)ngrid * grp));
     NGW_TRY(dbox.ensure(sizeof(double) * (size_t)boxsz * nb));
     NGW_TRY(dfbox.ensure(sizeof(double) * 2 * (size_t)csz * nb));
     NGW_TRY(dvc.ensure(sizeof(double) * (size_t)csz));
@@ -175,8 +181,10 @@ static int coulomb_impl(ngw_context* ctx, int ngrid, int n_intp, const double* z
     NGW_FFT(hipfftSetStream(pb, s));
     const double scale = 1.0 / ((double)n1 * (double)n2 * (double)n3);
     const double norm = ntrans / (h * h * h);
+    int g0 = 0;  // first column of the current group
     for (int c0 = 0; c0 < n_intp; c0 += nb) {
         const int cb = std::min(nb, n_intp - c0);
+        double* vz = dvz.as<double>() + (size_t)(c0 - g0) * ngrid;
         NGW_HIP(hipMemsetAsync(dbox.ptr, 0, sizeof(double) * (size_t)boxsz * nb, s));
         hipLaunchKernelGGL(scatter_kernel, dim3(std::min((ngrid + 255) / 256, 1024), cb), dim3(256), 0, s,
                            zd + (size_t)c0 * ldz, ldz, ngrid, doff.as<long>(), boxsz,
@@ -188,11 +196,15 @@ static int coulomb_impl(ngw_context* ctx, int ngrid, int n_intp, const double* z
         NGW_HIP(hipGetLastError());
         NGW_FFT(hipfftExecZ2D(pb, reinterpret_cast<hipfftDoubleComplex*>(dfbox.ptr), dbox.as<double>()));
         hipLaunchKernelGGL(gather_kernel, dim3(std::min((ngrid + 255) / 256, 1024), cb), dim3(256), 0, s,
-                           dbox.as<double>(), boxsz, doff.as<long>(), ngrid, scale, dvz.as<double>(), (long)ngrid);
+                           dbox.as<double>(), boxsz, doff.as<long>(), ngrid, scale, vz, (long)ngrid);
         NGW_HIP(hipGetLastError());
-        // M(:, c0:c0+cb) = norm * zeta^T vz
-        NGW_TRY(gemm_f64_tn(s, n_intp, cb, ngrid, norm, zd, ldz, dvz.as<double>(), ngrid,
-                            dm.as<double>() + (size_t)c0 * n_intp, n_intp));
+        const int gend = c0 + cb;
+        if (gend - g0 == grp || gend == n_intp) {
+            // M(:, g0:gend) = norm * zeta^T vz
+            NGW_TRY(gemm_f64_tn(s, n_intp, gend - g0, ngrid, norm, zd, ldz, dvz.as<double>(), ngrid,
+                                dm.as<double>() + (size_t)g0 * n_intp, n_intp));
+            g0 = gend;
+        }
     }
     std::vector<double> hm((size_t)n_intp * n_intp);
     NGW_HIP(hipMemcpyAsync(hm.data(), dm.ptr, sizeof(double) * hm.size(), hipMemcpyDeviceToHost, s));
