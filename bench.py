@@ -556,9 +556,13 @@ def main():
         ctx.set_tile_config(tiles[0], tiles[1])
         log(f"[bench] tile configurations given: K1 config {tiles[0]}, K3 config {tiles[1]}")
     elif args.precision != "f64" and not args.no_autotune:
+        # tuned at the timed launch width: every planned state's bands in one
+        # launch in served mode, one state's (or one chunk's) otherwise
+        n_plan = max(1, min(args.steps, len(order)))
+        tune_n = nb * n_plan if args.mode == "served" else (nb if args.band_chunk <= 0 else min(nb, args.band_chunk))
         t = time.time()
-        tiles = ctx.autotune(nb, reps=2)
-        log(f"[bench] autotune on {nb} vectors: K1 config {tiles[0]}, K3 config {tiles[1]} "
+        tiles = ctx.autotune(tune_n, reps=2)
+        log(f"[bench] autotune on {tune_n} vectors: K1 config {tiles[0]}, K3 config {tiles[1]} "
             f"({time.time() - t:.1f}s)")
 
     def newton(sig, sel):

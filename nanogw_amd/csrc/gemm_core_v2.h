@@ -247,8 +247,9 @@ struct SlabW {
 };
 
 // MFMA work on one LDS slab: the wave's (BM/2) x (BN/2) block of the tile.
-template <int KIND, int BM, int BN, int SWZ>
-__device__ __forceinline__ void mfma_slab(const SmemV2<KIND, BM, BN, SWZ>& sm, int cur, AccLp<BM, BN>& acc)
+// SM is SmemV2 or SmemDma (buffers a[cur][plane][row][k], b likewise).
+template <int KIND, int BM, int BN, int SWZ, class SM>
+__device__ __forceinline__ void mfma_slab(const SM& sm, int cur, AccLp<BM, BN>& acc)
 {
     typedef V2Traits<KIND> Tr;
     constexpr int FM = BM / 32, FN = BN / 32;
@@ -389,6 +390,131 @@ __device__ __forceinline__ void put_v2(typename V2Traits<KIND>::elem* hi, typena
         hi[i] = h;
         if constexpr (KIND == LP_BF16X3) lo[i] = f32_to_bf16_bits(v - bf16_bits_to_f32(h));
     }
+}
+
+// ---------------------------------------------------------------- direct-to-LDS slabs
+// gfx950's global_load_lds_dwordx4 moves 16 bytes per lane from global memory
+// straight into LDS: the 64 lanes of one instruction fill one 1 KB block, lane
+// l at byte 16 l (the block base goes in M0).  With 64-byte rows and the SWZ 1
+// chunk swizzle, block bi of an operand slab holds rows 16 bi .. 16 bi + 15,
+// and lane l fills row 16 bi + l / 4 at position l % 4, so it loads chunk
+// (l % 4) ^ ((row >> 1) & 3) of that row (lds_chunk is its own inverse).  No
+// register holds slab data and no ds_write is issued.  A slab is ready once
+// the loads of every wave have passed vmcnt and the workgroup has met at a
+// barrier; NB buffers keep NB - 1 slabs in flight.
+template <int KIND, int BM, int BN, int NB>
+struct alignas(16) SmemDma {
+    typedef V2Traits<KIND> Tr;
+    alignas(16) typename Tr::elem a[NB][Tr::NSPLIT][BM][Tr::BK];
+    alignas(16) typename Tr::elem b[NB][Tr::NSPLIT][BN][Tr::BK];
+};
+
+template <int KIND, int MN>
+struct DmaSlab {
+    typedef V2Traits<KIND> Tr;
+    static constexpr int PERW = MN / 64;            // instructions per wave per plane
+    static constexpr int NLOAD = PERW * Tr::NSPLIT; // per thread per slab
+    static_assert(MN % 64 == 0, "direct-to-LDS slab: rows a multiple of 64");
+    const char* src[Tr::NSPLIT][PERW];
+
+    __device__ __forceinline__ void init(const OperandV2<KIND>& op, int mn0)
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const long es = (long)sizeof(typename Tr::elem);
+        const int rows_left = op.rows - mn0;  // rows past the operand read its zero row
+#pragma unroll
+        for (int i = 0; i < PERW; ++i) {
+            const int row = 16 * (4 * i + wave) + (lane >> 2);
+            const int r = row < rows_left ? row : rows_left;
+            const long off = ((long)mn0 + r) * op.ld * es + 16 * lds_chunk<1>(row, lane & 3);
+            src[0][i] = reinterpret_cast<const char*>(op.hi) + off;
+            if constexpr (Tr::NSPLIT == 2) src[1][i] = reinterpret_cast<const char*>(op.lo) + off;
+        }
+    }
+    // loads of the slab at k0 into dst[plane][row][k]
+    __device__ __forceinline__ void issue(int k0, typename Tr::elem (*dst)[MN][Tr::BK]) const
+    {
+        const int wave = threadIdx.x >> 6;
+        const long koff = (long)k0 * (long)sizeof(typename Tr::elem);
+#pragma unroll
+        for (int sp = 0; sp < Tr::NSPLIT; ++sp)
+#pragma unroll
+            for (int i = 0; i < PERW; ++i)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(src[sp][i] + koff),
+                    (__attribute__((address_space(3))) void*)&dst[sp][16 * (4 * i + wave)][0], 16, 0, 0);
+    }
+};
+
+// s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding: vmcnt bits 3:0 and 15:14,
+// expcnt 6:4 left at its maximum, lgkmcnt 11:8).
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm0()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | ((N >> 4) << 14));
+}
+
+// Workgroup barrier that keeps the compiler from moving memory accesses
+// across it and adds no waits of its own (the caller has waited).
+__device__ __forceinline__ void slab_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// C tile (BM x BN) = A B^T over Kpad with direct-to-LDS slabs.  Slab t is read
+// from buffer t mod NB; at its turn each thread waits until only the newer
+// NB - 2 slabs' loads are outstanding (and its own LDS reads of the previous
+// slab are done), the workgroup meets, and the loads of slab t + NB - 1 go
+// into the buffer the previous slab occupied.  Requests past the last slab
+// re-read it.  On return no load is in flight and the LDS is free.
+template <int KIND, int BM, int BN, int NB>
+__device__ __forceinline__ void gemm_dma_tile(const OperandV2<KIND>& A, const OperandV2<KIND>& B, int m0, int n0,
+                                              int Kpad, AccLp<BM, BN>& acc, SmemDma<KIND, BM, BN, NB>& sm)
+{
+    static_assert(NB >= 2, "at least two buffers");
+    typedef V2Traits<KIND> Tr;
+    constexpr int FM = BM / 32, FN = BN / 32, BK = Tr::BK;
+    constexpr int L = DmaSlab<KIND, BM>::NLOAD + DmaSlab<KIND, BN>::NLOAD;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc.v[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    DmaSlab<KIND, BM> la;
+    DmaSlab<KIND, BN> lb;
+    la.init(A, m0);
+    lb.init(B, n0);
+    const int nslab = Kpad / BK;
+    const int last = (nslab - 1) * BK;
+    static_for<NB - 1>([&](auto sc) {
+        constexpr int u = decltype(sc)::value;
+        const int k = min(u * BK, last);
+        la.issue(k, sm.a[u]);
+        lb.issue(k, sm.b[u]);
+    });
+    auto slab = [&](int t, auto uc) {
+        constexpr int u = decltype(uc)::value;  // t mod NB
+        constexpr int w = (u + NB - 1) % NB;    // the previous slab's buffer
+        wait_vm_lgkm0<(NB - 2) * L>();
+        slab_barrier();
+        const int k = min((t + NB - 1) * BK, last);
+        la.issue(k, sm.a[w]);
+        lb.issue(k, sm.b[w]);
+        mfma_slab<KIND, BM, BN, 1>(sm, u, acc);
+    };
+    const int nfull = nslab - nslab % NB;
+    int t0 = 0;
+    for (; t0 < nfull; t0 += NB) {
+        static_for<NB>([&](auto uc) { slab(t0 + decltype(uc)::value, uc); });
+    }
+    const int rem = nslab - nfull;
+    static_for<NB - 1>([&](auto uc) {
+        if (decltype(uc)::value < rem) slab(t0 + decltype(uc)::value, uc);
+    });
+    wait_vm_lgkm0<0>();
+    slab_barrier();
 }
 
 }  // namespace ngw

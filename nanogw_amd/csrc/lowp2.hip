@@ -58,7 +58,8 @@ constexpr int L2K2 = 64;
 struct LpCfg {
     int bm, bn, d, pm;
     int fw = 0;  // K3 only: 1 = W formed in the slab loader (no wprep pass)
-    int sw = 0;  // LDS layout (lds_chunk): 0 padded rows, 1 XOR-swizzled rows
+    int sw = 0;  // slabs: 0 padded LDS rows, 1 XOR-swizzled rows (both register-prefetched),
+                 // 2 swizzled rows filled by direct-to-LDS loads into D buffers
 };
 constexpr int LP_NCFG = NGW_NTILE_CONFIGS;
 constexpr LpCfg K1_CFG[3][LP_NCFG] = {
@@ -66,7 +67,7 @@ constexpr LpCfg K1_CFG[3][LP_NCFG] = {
      {128, 128, 2, 2, 0, 1}, {128, 128, 3, 8, 0, 1}},                                             // FP32, BF16
     {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 64, 2, 8, 0, 1}, {128, 128, 1, 8, 0, 1},
      {128, 128, 2, 2, 0, 1}, {128, 64, 2, 2, 0, 1}},                                              // BF16x3
-    {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 128, 2, 4, 0, 1}, {256, 128, 2, 8, 0, 1},
+    {{128, 128, 2, 8, 0, 1}, {128, 128, 3, 8, 0, 2}, {128, 128, 2, 4, 0, 1}, {128, 128, 4, 8, 0, 2},
      {128, 128, 2, 2, 0, 1}, {128, 128, 3, 8, 0, 1}}};                                            // FP16
 // PM = 1 or 2 (indices 4, 5): each XCD covers all or half of the c rows, so W
 // is read once or twice instead of once per M-group.
@@ -75,7 +76,7 @@ constexpr LpCfg K3_CFG[3][LP_NCFG] = {
      {128, 128, 2, 2, 0, 1}, {128, 128, 2, 1, 0, 1}},
     {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {64, 128, 2, 8},
      {128, 128, 2, 2, 0, 1}, {128, 128, 2, 1, 0, 1}},
-    {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {192, 128, 3, 8, 0, 1}, {128, 192, 2, 4, 0, 1},
+    {{128, 128, 2, 4}, {128, 192, 3, 1, 0, 2}, {128, 192, 3, 4, 0, 2}, {128, 192, 2, 4, 0, 1},
      {128, 192, 2, 2, 0, 1}, {128, 192, 2, 1, 0, 1}}};  // FP16: W from wprep
 constexpr int NXCD = 8;
 __host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : kind == LP_F16 ? 2 : 0; }
@@ -523,13 +524,27 @@ __global__ __launch_bounds__(256) void red2_kernel(const LpOp* ops, LpOp op1, co
 }
 
 // ---------------------------------------------------------------- GEMMs
+// LDS of a K1 / K3 tile: SW 0 / 1 register-prefetched slabs (SmemV2, two
+// buffers, D slabs in flight), SW 2 direct-to-LDS slabs in D buffers with the
+// SWZ 1 row layout (SmemDma).
+template <int KIND, int BM, int BN, int D, int SW>
+using LpSmem = std::conditional_t<SW == 2, SmemDma<KIND, BM, BN, D>, SmemV2<KIND, BM, BN, SW == 2 ? 1 : SW>>;
+
+template <int KIND, int BM, int BN, int D, int SW>
+__device__ __forceinline__ void lp_gemm(const OperandV2<KIND>& A, const OperandV2<KIND>& B, int m0, int n0, int Kpad,
+                                        AccLp<BM, BN>& acc, LpSmem<KIND, BM, BN, D, SW>& sm)
+{
+    if constexpr (SW == 2) gemm_dma_tile<KIND, BM, BN, D>(A, B, m0, n0, Kpad, acc, sm);
+    else gemm_v2_tile<KIND, BM, BN, D>(A, B, m0, n0, Kpad, acc, sm);
+}
+
 template <int KIND, int BM, int BN, int D, int PM, int SW>
 __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
     constexpr int HALF = BM < 64 ? BM : 64;  // rows reduced per LDS pass (keeps the buffer within the slabs' LDS)
     __shared__ union {
-        SmemV2<KIND, BM, BN, SW> ab;
+        LpSmem<KIND, BM, BN, D, SW> ab;
         float red[HALF][BN + 1];
     } sm;
     LP_PICK_OP();
@@ -538,9 +553,9 @@ __global__ __launch_bounds__(GNT) void k1v2_kernel(const LpOp* ops, LpOp op1, co
     if (!xcd_rect_tile(local, (op.nmu + BM - 1) / BM, (N + BN - 1) / BN, PM, tm, tn)) return;
     const int m0 = tm * BM, n0 = tn * BN;
     AccLp<BM, BN> acc;
-    gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pct_hi, (const E*)op.pct_lo, op.ldc1, op.nmu},
-                                  OperandV2<KIND>{(const E*)op.xlp_hi, (const E*)op.xlp_lo, op.ldc1, N}, m0, n0,
-                                  op.ldc1, acc, sm.ab);
+    lp_gemm<KIND, BM, BN, D, SW>(OperandV2<KIND>{(const E*)op.pct_hi, (const E*)op.pct_lo, op.ldc1, op.nmu},
+                                 OperandV2<KIND>{(const E*)op.xlp_hi, (const E*)op.xlp_lo, op.ldc1, N}, m0, n0,
+                                 op.ldc1, acc, sm.ab);
     const int nend = min(n0 + BN, N);
     const int b_lo = n0 / nv, b_hi = (nend - 1) / nv;
     const int nbt = b_hi - b_lo + 1;
@@ -733,7 +748,7 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                                                    int nstage, int s, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
-    __shared__ SmemV2<KIND, BM, BN, SW> sm;
+    __shared__ LpSmem<KIND, BM, BN, D, SW> sm;
     LP_PICK_OP();
     const LpStage sg = ops ? st[r * nstage + s] : st1;
     const int nv = op.nv, N = nv * op.nvec;
@@ -748,9 +763,9 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
         lb.init(op.psiv32, op.u32, op.ldm1, nv, N, op.nvec, n0);
         gemm_v2_run<KIND, BM, BN, D>(la, lb, op.ldm1, acc, sm);
     } else {
-        gemm_v2_tile<KIND, BM, BN, D>(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc},
-                                      OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0,
-                                      n0, op.ldm1, acc, sm);
+        lp_gemm<KIND, BM, BN, D, SW>(OperandV2<KIND>{(const E*)op.pc_hi, (const E*)op.pc_lo, op.ldm1, op.nc},
+                                     OperandV2<KIND>{(const E*)op.wlp_hi, (const E*)op.wlp_lo, op.ldm1, N}, m0,
+                                     n0, op.ldm1, acc, sm);
     }
     // epilogue: OperatorEpilogue's stage (diagonal term, Horner / Clenshaw
     // combination, Lanczos accumulate), written out for one leading
