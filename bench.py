@@ -370,7 +370,8 @@ def run_stream(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     recs, st = serve_stream(ctx, mlist, reqs, niter, nen, deltae, ecuts, budget, sigx=sigx,
-                            use_graph=not args.no_graph, batch=batch, sids=sids, polys=polys, log=log)
+                            use_graph=not args.no_graph, batch=batch, sids=sids, polys=polys, log=log,
+                            large_factor=args.large_factor, large_every=args.large_every)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     solves = sum(len(r["qp"]) for r in reqs)
@@ -390,7 +391,10 @@ def run_stream(args):
                                f"requests, U{{4..16}} QP states each) over {len(mlist)} resident molecules "
                                f"(c4 distribution, large share {args.large_frac}); niter={niter}, npoly=12; "
                                f"continuous batching" + ("" if args.no_graph else ", hipGraph step replay"),
-                   "parallelism": "single GPU", "tile_configs": tiles},
+                   "parallelism": "single GPU", "tile_configs": tiles,
+                   "policy": (f"a request whose step costs more than {args.large_factor:g}x the median of the "
+                              f"others is held while smaller ones run, except every {args.large_every}th step"
+                              if args.large_every else "every request in every step")},
         "latency": lat, "serve": st, "gflops": fl / elapsed / 1e9,
         "operator_frac_of_peak": fl / elapsed / PEAK[args.precision],
         "roofline": None, "cpu_baseline": None,
@@ -473,6 +477,10 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="c5: arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="c5: seconds of arrivals")
     ap.add_argument("--large-frac", type=float, default=0.1, help="c5: share of R = 13, nv = 100 requests")
+    ap.add_argument("--large-every", type=int, default=8,
+                    help="c5: large requests step only every N-th step while smaller ones run (0: always)")
+    ap.add_argument("--large-factor", type=float, default=50.0,
+                    help="c5: step cost above this multiple of the others' median makes a request large")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch each step directly")
     ap.add_argument("--mode", default="served", choices=["served", "single", "single-vector"],
                     help="served / single: the QP-solve headline; single-vector: one-vector FP64 operator timing")

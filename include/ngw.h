@@ -386,7 +386,11 @@ int ngw_gemm_lowp_dev(ngw_context* ctx, int kind, int m, int n, int k, const dou
  *                  re-captured when requests join or leave); requests that
  *                  reach niter get their continued fraction and Sigma_c
  *   result         Sigma_c(nen, nqp) complex (Ry) once done
- * A request's result does not depend on the other requests in the batch.
+ *   hold           hold = 1 keeps an unfinished request out of the steps (its
+ *                  Lanczos state stays resident), hold = 0 lets it step again:
+ *                  the serving loop's lever for giving short requests the GPU
+ * A request's result does not depend on the other requests in the batch, nor
+ * on when it was held.
  */
 typedef struct ngw_batch ngw_batch;
 int ngw_batch_create(ngw_context* ctx, ngw_batch** batch);
@@ -402,6 +406,7 @@ int ngw_batch_step(ngw_batch* batch, int nsteps, int use_graph, int* n_finished)
 int ngw_batch_active(ngw_batch* batch, int* n_active);
 int ngw_batch_result(ngw_batch* batch, int req_id, double* sigma_c, int* done);
 int ngw_batch_release(ngw_batch* batch, int req_id);
+int ngw_batch_hold(ngw_batch* batch, int req_id, int hold);
 int ngw_batch_stats(ngw_batch* batch, long* steps_run, long* graph_captures);
 
 #ifdef __cplusplus

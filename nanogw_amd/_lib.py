@@ -75,6 +75,7 @@ _SIGS = {
     "ngw_batch_active": ([_vp, _ip], C.c_int),
     "ngw_batch_result": ([_vp, C.c_int, _dp, _ip], C.c_int),
     "ngw_batch_release": ([_vp, C.c_int], C.c_int),
+    "ngw_batch_hold": ([_vp, C.c_int, C.c_int], C.c_int),
     "ngw_batch_stats": ([_vp, C.POINTER(C.c_long), C.POINTER(C.c_long)], C.c_int),
     "ngw_gemm_f64_dev": ([_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_long, _vp, C.c_long, _vp, C.c_long],
                          C.c_int),

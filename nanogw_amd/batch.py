@@ -93,14 +93,42 @@ class GwBatch:
     def release(self, rid):
         _lib.check(self._lib.ngw_batch_release(self._h, int(rid)), "ngw_batch_release")
 
+    def hold(self, rid, hold=True):
+        """Keep an unfinished request out of the steps (hold=True) or let it
+        step again; its Lanczos state stays on the device."""
+        _lib.check(self._lib.ngw_batch_hold(self._h, int(rid), 1 if hold else 0), "ngw_batch_hold")
+
     def stats(self):
         s, g = C.c_long(), C.c_long()
         _lib.check(self._lib.ngw_batch_stats(self._h, C.byref(s), C.byref(g)), "ngw_batch_stats")
         return {"steps_run": s.value, "graph_captures": g.value}
 
 
+def step_cost(m, nqp):
+    """Operator FLOP of one Lanczos step of a request (nqp states, all bands
+    as columns, F_mv = 4 Nmu nv nc + 2 Nmu^2 per vector, npoly applications
+    cost the same for every request and are left out)."""
+    nmu, nb = m.psi_intp.shape[0], len(m.e1)
+    nc = nb - m.nv
+    return nqp * nb * (4.0 * nmu * m.nv * nc + 2.0 * nmu * nmu)
+
+
+def large_requests(costs, factor):
+    """Ids of the requests whose step cost exceeds `factor` times the median
+    step cost of the other running requests (none when only one runs)."""
+    if len(costs) < 2:
+        return set()
+    out = set()
+    for rid, c in costs.items():
+        others = [v for r, v in costs.items() if r != rid]
+        if c > factor * float(np.median(others)):
+            out.add(rid)
+    return out
+
+
 def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecuts, hbm_budget_bytes,
-                 sigx=None, use_graph=True, max_active=None, log=None, batch=None, sids=None, polys=None):
+                 sigx=None, use_graph=True, max_active=None, log=None, batch=None, sids=None, polys=None,
+                 large_factor=50.0, large_every=8):
     """Continuous-batching request loop.
 
     molecules: list of objects with psi_intp, mmtrx, e1, nv (resident systems)
@@ -108,6 +136,12 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
     batch, sids, polys: an existing GwBatch with the molecules already added
                (system ids and sqrt polynomials per molecule), so that operator
                upload and preparation stay outside a timed serving window
+    large_factor, large_every: a running request whose step costs more than
+               large_factor x the running median is held out of the batched
+               steps while smaller requests run, except on every large_every-th
+               step (it keeps progressing and cannot starve); with only large
+               requests running, they step every time.  large_every = 0 turns
+               the policy off (every request in every step).
     Returns per-request records with arrival, admit, finish times, Sigma_c and
     QP energies (Newton on the device), and loop statistics.
     """
@@ -120,6 +154,9 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
     pending = sorted(range(len(requests)), key=lambda i: requests[i]["t"])
     queue = []           # arrived, waiting for admission (FIFO by arrival)
     running = {}         # rid -> request index
+    cost = {}            # rid -> operator FLOP per Lanczos step
+    held = set()
+    held_steps = 0
     used = 0
     recs = [None] * len(requests)
     s0 = batch.stats()["steps_run"]
@@ -143,12 +180,22 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
             heapq.heappop(queue)
             rid = batch.submit(sids[rq["mol"]], rq["qp"], nen, deltae, ecuts, niter, polys[rq["mol"]])
             running[rid] = (i, need)
+            cost[rid] = step_cost(molecules[rq["mol"]], len(rq["qp"]))
             used += need
             recs[i] = {"arrival": rq["t"], "admit": time.perf_counter() - t0, "mol": rq["mol"], "qp": rq["qp"]}
         if not running:
             if pi < len(pending):
                 time.sleep(max(0.0, min(0.01, requests[pending[pi]]["t"] - now)))
             continue
+        if large_every:
+            large = large_requests({rid: cost[rid] for rid in running}, large_factor)
+            run_large = len(large) == len(running) or steps % large_every == 0
+            want = set() if run_large else large
+            for rid in running:
+                if (rid in want) != (rid in held):
+                    batch.hold(rid, rid in want)
+            held = want
+            held_steps += 1 if want else 0
         batch.step(1, use_graph)
         steps += 1
         if log and time.perf_counter() - t_log > 20.0:
@@ -169,12 +216,15 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
                             "zfac": res["zfac"], "ierr": res["ierr"]})
             batch.release(rid)
             del running[rid]
+            cost.pop(rid, None)
+            held.discard(rid)
             used -= need
             done += 1
             if log:
                 log(f"[serve] request {i} done: latency {recs[i]['finish'] - recs[i]['arrival']:.3f}s")
     stats = batch.stats()
     stats["batch_steps"] = steps
+    stats["steps_with_large_held"] = held_steps
     stats["lanczos_steps"] = stats["steps_run"] - s0
     stats["wall_s"] = time.perf_counter() - t0
     if own:

@@ -256,6 +256,7 @@ struct ngw_batch {
         DevBuf vec, work, lpwork, ab, zz, spec, misc, kstep, sig;
         int k_host = 1;  // next Lanczos step to run
         bool done = false;
+        bool held = false;  // kept out of the steps (ngw_batch_hold), state intact
         std::vector<double> sigma;
         size_t bytes = 0;
     };
@@ -363,7 +364,7 @@ int rebuild_group(ngw_batch* b)
     drop_graph(b);
     b->active.clear();
     for (auto& kv : b->reqs)
-        if (!kv.second.done) b->active.push_back(kv.first);
+        if (!kv.second.done && !kv.second.held) b->active.push_back(kv.first);
     const int nreq = (int)b->active.size();
     b->dirty = false;
     if (nreq == 0) return NGW_OK;
@@ -693,6 +694,17 @@ int ngw_batch_release(ngw_batch* b, int req_id)
     NGW_HIP(hipStreamSynchronize(b->ctx->stream));
     if (!it->second.done) b->dirty = true;
     b->reqs.erase(it);
+    return NGW_OK;
+}
+
+int ngw_batch_hold(ngw_batch* b, int req_id, int hold)
+{
+    if (!b) return fail(NGW_ERR_ARG, "null batch");
+    auto it = b->reqs.find(req_id);
+    if (it == b->reqs.end()) return fail(NGW_ERR_ARG, "unknown request");
+    const bool h = hold != 0;
+    if (it->second.held != h && !it->second.done) b->dirty = true;
+    it->second.held = h;
     return NGW_OK;
 }
 

@@ -19,6 +19,42 @@ def mols():
             synth.make_molecule(5.0, 0.55, 5, 20, 56, seed=4242, ridge=1e-8)]
 
 
+def test_held_request_equals_alone(ngw_ctx, mols):
+    """A request held out of the steps for a while (ngw_batch_hold) and then
+    resumed gives the same Sigma_c, bit for bit, as run alone; a held request
+    does not step and the others run past it."""
+    from nanogw_amd.batch import GwBatch
+    from nanogw_amd.lanczos import lanczos_polynomial
+
+    niter, nen, deltae, ecuts = 20, 21, 4.0, 0.02
+    ngw_ctx.set_precision("f64")
+    b = GwBatch(ngw_ctx)
+    try:
+        sids = [b.add_system(m.psi_intp, m.mmtrx, m.e1, m.nv) for m in mols[:2]]
+        polys = [lanczos_polynomial(m.e1, m.nv, 12) for m in mols[:2]]
+        qps = [[m.nv - 1, m.nv] for m in mols[:2]]
+        alone = []
+        for k in range(2):
+            rid = b.submit(sids[k], qps[k], nen, deltae, ecuts, niter, polys[k])
+            while b.result(rid, nen, 2) is None:
+                b.step(1, True)
+            alone.append(b.result(rid, nen, 2))
+            b.release(rid)
+        r0 = b.submit(sids[0], qps[0], nen, deltae, ecuts, niter, polys[0])
+        r1 = b.submit(sids[1], qps[1], nen, deltae, ecuts, niter, polys[1])
+        b.step(3, True)
+        b.hold(r1, True)
+        b.step(niter, True)          # r0 finishes while r1 waits
+        assert b.result(r0, nen, 2) is not None and b.result(r1, nen, 2) is None
+        b.hold(r1, False)
+        while b.result(r1, nen, 2) is None:
+            b.step(1, True)
+        assert np.array_equal(b.result(r0, nen, 2), alone[0])
+        assert np.array_equal(b.result(r1, nen, 2), alone[1])
+    finally:
+        b.close()
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_batch_matches_oracle_and_alone(ngw_ctx, mols, use_graph):
     from nanogw_amd.batch import GwBatch
