@@ -266,6 +266,7 @@ std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const do
             st.flags |= ST_X32;
         } else {
             st.x = q; st.alpha = f * bk1.qc / al; gam += -f * be / al * bk1.qc;
+            st.flags |= ST_QW32;  // the first stage: its x' pass keeps q as FP32
         }
         if (bk2.p) {
             st.y = bk2.p; st.by = -1.0;
@@ -284,7 +285,7 @@ std::vector<PolyStage> plan_poly(const ngw_context* ctx, int precision, const do
             nb = (nb + 1) % 3;
             if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
             if (o == bk1.p || o == bk2.p) { o = bufs[nb]; nb = (nb + 1) % 3; }
-            st.flags |= ST_O32;
+            st.flags |= ST_O32 | ST_Q32;  // an FP32 b_k: its gamma q term reads q in FP32
         }
         st.out = o;
         plan.push_back(st);

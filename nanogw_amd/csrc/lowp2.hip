@@ -368,6 +368,7 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
     LP_PICK_OP();
     const LpStage& sg = ops ? st[r * nstage + s] : st1;
     const bool x32 = sg.flags & ST_X32;
+    const bool wq32 = !x32 && (sg.flags & ST_QW32) && op.q32;  // x is q: keep an FP32 copy
     const int vt = xp_vt(op.nv), ct = xp_ct(op.nv, op.ldc1);
     const int nvt = (op.nv + vt - 1) / vt, nct = (op.ldc1 + ct - 1) / ct;
     const int b = local / (nvt * nct);
@@ -412,6 +413,12 @@ __global__ __launch_bounds__(256) void xprep_kernel(const LpOp* ops, LpOp op1, c
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (e0 + 256 * u < tot) tile[cc[u] * pitch + vv[u]] = (float)(xv[u] * (sv[u] * f16s));
+            if (wq32) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (e0 + 256 * u < tot)
+                        op.q32[(long)b * op.nv * op.nc + base + vv[u] + (long)cc[u] * op.nv] = (float)xv[u];
+            }
         }
     };
     // FP32 input over whole v columns (vn == nv) is one contiguous run of x
@@ -774,6 +781,9 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
     const double beta = op.tda ? 2.0 : 4.0;
     const bool acc_on = sg.accumulate == 1 || (sg.accumulate == 2 && *op.kstep > 1);
     const bool use_q = op.q != nullptr && sg.gamma != 0.0;
+    // FP32 output stages read the first stage's FP32 copy of q
+    const float* q32 = (sg.flags & ST_Q32) && (sg.flags & ST_O32) ? op.q32 : nullptr;
+    const long ncv = (long)nv * op.nc;
     const float* sigc = op.sig + nv;
     // element types of x, y, out fixed at compile time per flag combination
     auto epi = [&](auto xt, auto yt, auto ot) {
@@ -787,7 +797,8 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
         // multiple of four (tiles start at multiples of four): then every
         // array is read and written with 16-byte accesses
         const bool vec = (nv & 3) == 0 && (op.ld & 3) == 0 && aligned16(xp) && aligned16(yp) && aligned16(op_) &&
-                         aligned16(use_q ? op.q : nullptr) && aligned16(dp) && (KIND != LP_F16 || aligned16(op.wsc));
+                         aligned16(use_q ? op.q : nullptr) && aligned16(q32) && aligned16(dp) &&
+                         (KIND != LP_F16 || aligned16(op.wsc)) && (ncv & 3) == 0;
         if (!vec) {
             for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
                 const int c = m0 + row, j = n0 + col;
@@ -801,7 +812,7 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                     const double hx = (double)f.x * xv + (beta * (double)f.y) * Y;
                     double r = sg.alpha * hx + sg.bx * xv;
                     if (yp) r = r + sg.by * (double)yp[i];
-                    if (use_q) r = r + sg.gamma * op.q[i];
+                    if (use_q) r = r + sg.gamma * (q32 ? (double)q32[idx + (long)b * ncv] : op.q[i]);
                     if (acc_on) r = (double)op_[i] + r;
                     op_[i] = (TO)r;
                 }
@@ -873,7 +884,8 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                     for (int u = 0; u < 4; ++u) res[u] = res[u] + sg.by * t4[u];
                 }
                 if (use_q) {
-                    ld4(op.q + i, t4);
+                    if (q32) ld4(q32 + idx + (long)b * ncv, t4);
+                    else ld4(op.q + i, t4);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) res[u] = res[u] + sg.gamma * t4[u];
                 }
@@ -1016,7 +1028,7 @@ size_t lp2_work_bytes(int kind, int nmu, int nv, int nc, int nvec)
     const size_t ldc1 = padk(nc), ldm1 = padk(nmu), N = (size_t)nv * nvec;
     return ns * (align256(es * (N + 1) * ldc1) + align256(es * (N + 1) * ldm1)) +
            2 * align256(4 * (size_t)(nvec + 1) * ldm1) + 2 * align256(4 * (size_t)nvec) + align256(4 * N) +
-           align256(4 * (size_t)nvec * XS_SPLIT);
+           align256(4 * (size_t)nvec * XS_SPLIT) + align256(4 * (size_t)nv * nc * nvec);
 }
 
 // Fill the descriptor of one request from its operator and workspaces.
@@ -1060,6 +1072,8 @@ void lp2_fill(const ngw_context* op, int kind, int nvec, long ld, int tda, char*
     d.vsc = f16 ? op->psiv_scale.as<float>() : nullptr;
     d.wsc = f16 ? reinterpret_cast<float*>(s0 + 2 * align256(4 * (size_t)nvec)) : nullptr;
     d.xpart = f16 ? reinterpret_cast<float*>(s0 + 2 * align256(4 * (size_t)nvec) + align256(4 * N)) : nullptr;
+    d.q32 = reinterpret_cast<float*>(s0 + 2 * align256(4 * (size_t)nvec) + align256(4 * N) +
+                                     align256(4 * (size_t)nvec * XS_SPLIT));
     d.tpart = tpart;
     d.q = nullptr; d.w = nullptr; d.a = nullptr; d.b = nullptr; d.niter = 0; d.kstep = nullptr;
 }

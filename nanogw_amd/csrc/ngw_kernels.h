@@ -18,8 +18,11 @@ __device__ __forceinline__ void vstore(double* p, bool f32, long i, double v)
     else p[i] = v;
 }
 
-// Stage flags: which vector blocks of a stage are stored as FP32.
-enum { ST_X32 = 1, ST_Y32 = 2, ST_O32 = 4 };
+// Stage flags: which vector blocks of a stage are stored as FP32.  ST_QW32:
+// the stage's input x is the polynomial's argument q (FP64), and the
+// low-precision x' pass also writes q as FP32 (LpOp::q32); ST_Q32: the
+// epilogue's gamma q term reads that FP32 copy (an FP32 output stage).
+enum { ST_X32 = 1, ST_Y32 = 2, ST_O32 = 4, ST_QW32 = 8, ST_Q32 = 16 };
 
 // Epilogue of the operator's last GEMM (K3), applied per output element
 // (pair index idx of vector b, GEMM value Y = (C^T M C x')(idx)):
@@ -158,6 +161,9 @@ struct LpOp {
     // per-band scales of Psi_v (vsc), scale of W row j = v + nv b
     // (wsc = vsc(v) x scale of u(:, b))
     float* xsc; unsigned* umax; const float* vsc; float* wsc; float* xpart;
+    // FP32 copy of q (pairs x vectors, leading dimension nv nc) for the
+    // gamma q term of the Clenshaw stages whose outputs are FP32
+    float* q32;
     double* q; double* w; double* a; double* b; int niter; int* kstep;
 };
 
