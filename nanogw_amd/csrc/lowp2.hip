@@ -651,61 +651,55 @@ __global__ __launch_bounds__(GNT) void k2v2_kernel(const LpOp* ops, LpOp op1, co
 }
 
 // W(j = v + nv b, mu) = Psi_v(mu, v) u(mu, b) for rows j <= N (row N zero),
-// in chunks of eight consecutive mu from the FP32 psiv32 / u rows; a
-// workgroup covers WP_CH consecutive chunks, each thread WP_CH / 256 of them
-// 256 chunks apart, with all loads issued before the first store.
-constexpr int WP_CH = 1024;
+// in chunks of eight consecutive mu from the FP32 psiv32 / u rows.  A
+// workgroup takes one vector b, WP_V consecutive v and 256 chunks of mu: each
+// thread loads its u chunk once and writes it into the WP_V rows v + nv b
+// (Psi_v rows come from L2), so every store instruction of a wave writes 1 KB
+// of one row.  Row N is the zero row: b = nvec, v = 0, u row nvec is zero.
+constexpr int WP_V = 8;
+__host__ __device__ constexpr int wp_grid(int nv, int nvec, int ldm1)
+{
+    return (nvec + 1) * ((nv + WP_V - 1) / WP_V) * ((ldm1 / 8 + 255) / 256);
+}
 template <int KIND>
 __global__ __launch_bounds__(256) void wprep_kernel(const LpOp* ops, LpOp op1, const int* pre, int nreq)
 {
     typedef typename V2Traits<KIND>::elem E;
-    constexpr int Q = WP_CH / 256;
     LP_PICK_OP();
-    const int per_row = op.ldm1 / 8, N = op.nv * op.nvec;
-    const long nchunk = (long)(N + 1) * per_row;
-    const bool small = nchunk < (1L << 31);  // 32-bit index division (64-bit division is a long VALU sequence)
-    f32x4 w0[Q], w1[Q];
-    float sc[Q];
-    int jq[Q], muq[Q];
+    const int per_row = op.ldm1 / 8, nmc = (per_row + 255) / 256, nvg = (op.nv + WP_V - 1) / WP_V;
+    const int mc = local % nmc, rest = local / nmc;
+    const int vg = rest % nvg, b = rest / nvg;
+    const int chunk = mc * 256 + threadIdx.x;
+    const int v0 = vg * WP_V;
+    const bool zero_row = b == op.nvec;  // only its v = 0 row exists (j = N)
+    if (chunk >= per_row || b > op.nvec || (zero_row && v0 > 0)) return;
+    const int mu = chunk * 8;
+    const float* u = op.u32 + (long)b * op.ldm1 + mu;
+    const f32x4 u0 = gload<f32x4>(u), u1 = gload<f32x4>(u + 4);
+    float usc = 1.0f;
+    if constexpr (KIND == LP_F16) usc = zero_row ? 1.0f : pow2_ceil(__uint_as_float(op.umax[b]));
+    const int vend = zero_row ? 1 : min(WP_V, op.nv - v0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const long i = (long)local * WP_CH + threadIdx.x + 256 * q;
-        int j;
-        if (small) j = (int)((unsigned)i / (unsigned)per_row);
-        else j = (int)(i / per_row);
-        const int mu = (int)(i - (long)j * per_row) * 8;
-        jq[q] = j;
-        muq[q] = mu;
-        w0[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-        w1[q] = w0[q];
-        sc[q] = 1.0f;
-        if (i < nchunk && j < N) {
-            const int b = j / op.nv, v = j - b * op.nv;
-            const float* p = op.psiv32 + (long)v * op.ldm1 + mu;
-            const float* u = op.u32 + (long)b * op.ldm1 + mu;
-            w0[q] = gload<f32x4>(p) * gload<f32x4>(u);
-            w1[q] = gload<f32x4>(p + 4) * gload<f32x4>(u + 4);
-            if constexpr (KIND == LP_F16) sc[q] = op.vsc[v] * pow2_ceil(__uint_as_float(op.umax[b]));
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const long i = (long)local * WP_CH + threadIdx.x + 256 * q;
-        if (i >= nchunk) break;
-        const int j = jq[q], mu = muq[q];
+    for (int dv = 0; dv < WP_V; ++dv) {
+        if (dv >= vend) break;
+        const int v = v0 + dv, j = v + op.nv * b;
+        const float* p = op.psiv32 + (long)v * op.ldm1 + mu;
+        f32x4 w0 = gload<f32x4>(p) * u0;
+        f32x4 w1 = gload<f32x4>(p + 4) * u1;
         if constexpr (KIND == LP_F16) {
-            const float f = F16_VEC / sc[q];
-            w0[q] = w0[q] * f;
-            w1[q] = w1[q] * f;
-            if (mu == 0 && j < N) op.wsc[j] = sc[q];
+            const float sc = zero_row ? 1.0f : op.vsc[v] * usc;
+            const float f = F16_VEC / sc;
+            w0 = w0 * f;
+            w1 = w1 * f;
+            if (mu == 0 && !zero_row) op.wsc[j] = sc;
         }
         const long o = (long)j * op.ldm1 + mu;
         if constexpr (KIND == LP_F32) {
             f32x4* out = reinterpret_cast<f32x4*>(reinterpret_cast<float*>(op.wlp_hi) + o);
-            out[0] = w0[q];
-            out[1] = w1[q];
+            out[0] = w0;
+            out[1] = w1;
         } else {
-            const float w[8] = {w0[q].x, w0[q].y, w0[q].z, w0[q].w, w1[q].x, w1[q].y, w1[q].z, w1[q].w};
+            const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
             unsigned hi[4], lo[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -923,7 +917,7 @@ static LpGrid lp_grid(int prec, int c1, int c3, int nmu, int nv, int nc, int nve
     g.k1 = xcd_grid((nmu + k1.bm - 1) / k1.bm, (nv * nvec + k1.bn - 1) / k1.bn, k1.pm);
     g.red = (int)(((long)(nvec + 1) * ldm1 + 255) / 256);
     g.k2 = ((nmu + L2K2 - 1) / L2K2) * ((nvec + L2K2 - 1) / L2K2);
-    g.wprep = (int)(((long)(nv * nvec + 1) * (ldm1 / 8) + WP_CH - 1) / WP_CH);
+    g.wprep = wp_grid(nv, nvec, ldm1);
     g.k3 = xcd_grid((nc + k3.bm - 1) / k3.bm, (nv * nvec + k3.bn - 1) / k3.bn, k3.pm);
     return g;
 }
