@@ -51,6 +51,17 @@ def build() -> str:
     return os.path.join(_HERE, "libngw_oracle.so")
 
 
+def build_ref() -> bool:
+    """Where the reference sources are present, compile NanoGW's own Fortran
+    type modules into oracle/_ref/fortran and the drop-in against them
+    (make target dropin-types); False where they are absent."""
+    r = subprocess.run(["make", "-s", "-C", _HERE, "ref-present"], capture_output=True, text=True)
+    if r.stdout.strip() != "yes":
+        return False
+    subprocess.run(["make", "-s", "-C", _HERE, "dropin-types"], check=True)
+    return True
+
+
 def lib():
     global _LIB
     if _LIB is None:
