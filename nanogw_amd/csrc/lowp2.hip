@@ -68,16 +68,22 @@ constexpr LpCfg K1_CFG[3][LP_NCFG] = {
     {{128, 128, 2, 8, 0, 1}, {128, 128, 2, 8}, {128, 64, 2, 8, 0, 1}, {128, 128, 1, 8, 0, 1},
      {128, 128, 2, 2, 0, 1}, {128, 64, 2, 2, 0, 1}},                                              // BF16x3
     {{128, 128, 2, 8, 0, 1}, {128, 128, 3, 8, 0, 2}, {128, 128, 2, 4, 0, 1}, {128, 128, 4, 8, 0, 2},
-     {128, 128, 2, 2, 0, 1}, {128, 128, 3, 8, 0, 1}}};                                            // FP16
-// PM = 1 or 2 (indices 4, 5): each XCD covers all or half of the c rows, so W
-// is read once or twice instead of once per M-group.
+     {128, 256, 3, 8, 0, 2}, {128, 128, 3, 8, 0, 1}}};                                            // FP16
+// PM = 1 or 2 (indices 4, 5 of the FP32/BF16 and BF16x3 sets): each XCD covers
+// all or half of the c rows, so W is read once or twice instead of once per
+// M-group.  FP16 K1 index 4 and K3 indices 4 and 5 are 128 x 256 tiles: per
+// slab a workgroup moves 24 KB into LDS and its four 64 x 128 wave tiles read
+// 48 KB for 128 MFMAs, 576 LDS bytes per MFMA against 768 for 128 x 128 and
+// 640 for 128 x 192.  Which tile wins depends on the shape; K3's result does
+// not (each element sums its K range in the same order in every tile), K1's
+// does through its per-BN partial-sum slots.
 constexpr LpCfg K3_CFG[3][LP_NCFG] = {
     {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {128, 128, 2, 8, 1, 1},
      {128, 128, 2, 2, 0, 1}, {128, 128, 2, 1, 0, 1}},
     {{128, 128, 2, 4}, {128, 128, 2, 4, 0, 1}, {128, 128, 2, 4, 1}, {64, 128, 2, 8},
      {128, 128, 2, 2, 0, 1}, {128, 128, 2, 1, 0, 1}},
     {{128, 128, 2, 4}, {128, 192, 3, 1, 0, 2}, {128, 192, 3, 4, 0, 2}, {128, 192, 2, 4, 0, 1},
-     {128, 192, 2, 2, 0, 1}, {128, 192, 2, 1, 0, 1}}};  // FP16: W from wprep
+     {128, 256, 2, 4, 0, 2}, {128, 256, 3, 1, 0, 2}}};  // FP16: W from wprep
 constexpr int NXCD = 8;
 __host__ __device__ constexpr int cfg_set(int kind) { return kind == LP_BF16X3 ? 1 : kind == LP_F16 ? 2 : 0; }
 static int cfg_set_prec(int prec) { return prec == PREC_BF16X3 ? 1 : prec == PREC_F16 ? 2 : 0; }
@@ -793,35 +799,17 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
         const bool vec = (nv & 3) == 0 && (op.ld & 3) == 0 && aligned16(xp) && aligned16(yp) && aligned16(op_) &&
                          aligned16(use_q ? op.q : nullptr) && aligned16(q32) && aligned16(dp) &&
                          (KIND != LP_F16 || aligned16(op.wsc)) && (ncv & 3) == 0;
-        if (!vec) {
-            for_acc_lp<BM, BN>(acc, [&](int row, int col, float val) {
-                const int c = m0 + row, j = n0 + col;
-                if (c < op.nc && j < N) {
-                    const int b = j / nv, v = j - b * nv;
-                    const long idx = v + (long)c * nv, i = idx + (long)b * op.ld;
-                    double Y = (double)val * (double)sigc[c];
-                    if constexpr (KIND == LP_F16) Y *= (double)op.wsc[j] * F16_UNSCALE;
-                    const float2 f = dp[idx];
-                    const double xv = (double)xp[i];
-                    const double hx = (double)f.x * xv + (beta * (double)f.y) * Y;
-                    double r = sg.alpha * hx + sg.bx * xv;
-                    if (yp) r = r + sg.by * (double)yp[i];
-                    if (use_q) r = r + sg.gamma * (q32 ? (double)q32[idx + (long)b * ncv] : op.q[i]);
-                    if (acc_on) r = (double)op_[i] + r;
-                    op_[i] = (TO)r;
-                }
-            });
-            return;
-        }
         // The tile passes through LDS one accumulator fragment row at a time:
         // in pass q every wave writes its fragment row q (16 tile rows, so
         // 32 rows in all: local rows 0-15 from the upper half of the tile,
         // 16-31 from the lower), then every thread takes runs of four
         // consecutive columns of one row (a wave covers whole rows, so each
         // access of the wave is one contiguous run of x, y, q and out per
-        // pair block).  The fragment index is a compile-time constant, so a
-        // pass frees its accumulators.  Pitch = BN + 16 floats: the 16 lanes
-        // of an accumulator row and the next row's 16 fall on disjoint banks.
+        // pair block), or single elements where the arrays do not allow
+        // 16-byte accesses.  The fragment index is a compile-time constant,
+        // so a pass frees its accumulators.  Pitch = BN + 16 floats: the 16
+        // lanes of an accumulator row and the next row's 16 fall on disjoint
+        // banks.
         constexpr int FM = BM / 32, EPR = 32, PITCH = BN + 16, G = BN / 4, PER = EPR * G / GNT;
         static_assert(EPR * G % GNT == 0, "epilogue row blocks");
         static_assert(sizeof(sm) >= sizeof(float) * EPR * PITCH, "epilogue staging exceeds the slab LDS");
@@ -837,9 +825,30 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                 for (int r = 0; r < 4; ++r)
                     stg[hrow + 4 * (lane >> 4) + r][wn + 16 * jf + (lane & 15)] = acc.v[q][jf][r];
             __syncthreads();
+            if (!vec) {
+#pragma unroll 1
+                for (int e = threadIdx.x; e < EPR * BN; e += GNT) {
+                    const int rl = e / BN, col = e - rl * BN;
+                    const int c = m0 + (rl < 16 ? 16 * q + rl : BM / 2 + 16 * q + rl - 16), j = n0 + col;
+                    if (c >= op.nc || j >= N) continue;
+                    const int b = j / nv, v = j - b * nv;
+                    const long idx = v + (long)c * nv, i = idx + (long)b * op.ld;
+                    double Y = (double)stg[rl][col] * (double)sigc[c];
+                    if constexpr (KIND == LP_F16) Y *= (double)op.wsc[j] * F16_UNSCALE;
+                    const float2 f = dp[idx];
+                    const double xv = (double)xp[i];
+                    const double hx = (double)f.x * xv + (beta * (double)f.y) * Y;
+                    double r = sg.alpha * hx + sg.bx * xv;
+                    if (yp) r = r + sg.by * (double)yp[i];
+                    if (use_q) r = r + sg.gamma * (q32 ? (double)q32[idx + (long)b * ncv] : op.q[i]);
+                    if (acc_on) r = (double)op_[i] + r;
+                    op_[i] = (TO)r;
+                }
+                return;
+            }
             // two groups in flight where the tile runs at two waves per SIMD
             // (its register budget has room), one at three waves per SIMD
-            constexpr int GUNROLL = BM * BN <= 128 * 128 ? 1 : 2;
+            constexpr int GUNROLL = BM * BN <= 128 * 128 || BM * BN > 128 * 192 ? 1 : 2;
 #pragma unroll GUNROLL
             for (int k = 0; k < PER; ++k) {
                 const int e = threadIdx.x + GNT * k;

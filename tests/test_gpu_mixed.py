@@ -96,10 +96,12 @@ def test_tile_configs_and_autotune_vs_oracle(ngw_ctx, tall, prec, tol):
             assert err < tol, (cfg, err)
             outs.append(y)
         if prec == "f16":
-            # configurations 1 and 2 (K3) and 1 and 3 (K1) fill their LDS slabs
-            # by direct-to-LDS loads; their tiles are the same shapes as
-            # configurations 5 and 3 (K3) and 5 and 2 (K1), whose slabs pass
-            # through registers, so the products are the same bit for bit
+            # K1 configurations 1 and 3 fill their LDS slabs by direct-to-LDS
+            # loads; configurations 5 and 2 have the same tile widths with
+            # slabs that pass through registers.  K3 configuration 2 is direct
+            # to LDS and 3 its register twin, and K3's products do not depend
+            # on the tile (1 and 5 are 128 x 192 and 128 x 256, both direct to
+            # LDS).  So the products are the same bit for bit
             for a, b in ((1, 5), (3, 2)):
                 print("direct-to-LDS vs register slabs", a, b, np.abs(outs[a] - outs[b]).max())
                 assert np.array_equal(outs[a], outs[b]), (a, b)
