@@ -137,11 +137,12 @@ def serve_stream(ctx: IsdfContext, molecules, requests, niter, nen, deltae, ecut
                (system ids and sqrt polynomials per molecule), so that operator
                upload and preparation stay outside a timed serving window
     large_factor, large_every: a running request whose step costs more than
-               large_factor x the running median is held out of the batched
-               steps while smaller requests run, except on every large_every-th
-               step (it keeps progressing and cannot starve); with only large
-               requests running, they step every time.  large_every = 0 turns
-               the policy off (every request in every step).
+               large_factor x the median step cost of the other running
+               requests is held out of the batched steps while smaller
+               requests run, except on every large_every-th step (it keeps
+               progressing and cannot starve); with only large requests
+               running, they step every time.  large_every = 0 turns the
+               policy off (every request in every step).
     Returns per-request records with arrival, admit, finish times, Sigma_c and
     QP energies (Newton on the device), and loop statistics.
     """
