@@ -19,6 +19,10 @@ Pinning (tests/test_oracle_*.py):
     isdf_kernel: absolute scale (ntrans/h^3, e^2 = 2) from the pair Coulomb
     energies of Gaussian zeta columns at two grid spacings and two ntrans,
     symmetry, positive semi-definiteness;
+    sigma_x_direct: -sum_n (mn|nm) of normalised Gaussian orbitals in closed
+    form (erf-form self energies of the Gaussian pair densities); in the
+    exact-ISDF limit (every grid point an interpolation point) the ISDF
+    exchange -sum_v C^T M C equals it to rounding;
   * spline/Newton: exactness on a cubic table away from the ends (the end
     slopes are finite differences), one-step Newton on linear f;
   * polyfit: numpy.linalg.lstsq on well-conditioned cases.  The npoly = 12
@@ -397,6 +401,29 @@ def isdf_kernel(zeta, gidx, nfft, h, trunc, ntrans=1):
     """M = (ntrans/h^3) zeta^T (V zeta), isdf_parallel_sym_UltraLowMem.F90:312, 749, 865-867."""
     vz = poisson(zeta, gidx, nfft, h, trunc)
     return (ntrans / h ** 3) * (zeta.T @ vz)
+
+
+TOL_OCC = 0.01  # shared/myconstants.F90:51
+
+
+def sigma_x_direct(psi, occ, gidx, nfft, h, trunc, qp, ntrans=1, nk=1):
+    """Bare exchange on the grid, SIGMA/fock_exchange.F90z (0D, real orbitals,
+    one spin and k-point): the occupied states are those with occ1 > tol_occ
+    (:107-111); for each state m and occupied n the pair density
+    rho = psi_n psi_m goes through Zpoisson (:117-158), and the kernel
+    -<psi_m | psi_n V[rho]> ntrans/h^3 (fac, :51, :174) is added with weight
+    occ1(n)/nk (:175-176).  psi: (ngrid, nb) with sum_r psi^2 = 1 (the
+    reference's wave-function normalisation); occ: (nb,).  Returns
+    Sigma_x(m) in Ry for m in qp."""
+    vc = coulomb_0d(nfft, h, trunc)
+    occ = np.asarray(occ, np.float64)
+    nocc = [n for n in range(psi.shape[1]) if occ[n] > TOL_OCC]
+    out = []
+    for m in qp:
+        rho = psi[:, nocc] * psi[:, [m]]
+        v = poisson(rho, gidx, nfft, h, trunc, vcoul=vc)
+        out.append(-(ntrans / h ** 3) * float(np.sum((rho * v).sum(axis=0) * occ[nocc] / nk)))
+    return np.array(out)
 
 
 def _snap_one(table, lo, taken, c):

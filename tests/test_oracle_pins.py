@@ -354,6 +354,73 @@ def test_isdf_kernel_scale_gaussian_pair_energies(O, h, ntrans):
     assert np.abs(M / ref - 1.0).max() < 1e-8
 
 
+def _gaussian_orbitals(g, h, cen, alpha):
+    r = g * h
+    phi = np.stack([np.exp(-a * ((r - c) ** 2).sum(1)) for c, a in zip(cen, alpha)], axis=1)
+    return np.asfortranarray(phi / np.sqrt((phi ** 2).sum(axis=0)))  # sum_r psi^2 = 1
+
+
+@pytest.mark.parametrize("h", [0.25, 0.3])
+def test_sigma_x_direct_gaussian_closed_form(O, h):
+    """sigma_x_direct (SIGMA/fock_exchange.F90z) on normalised Gaussian
+    orbitals phi_i = (2 a_i / pi)^(3/4) exp(-a_i |r - A_i|^2): the pair
+    density phi_m phi_n is the Gaussian K exp(-p |r - P|^2), p = a_m + a_n,
+    K = N_m N_n exp(-a_m a_n |A_m - A_n|^2 / p), whose Coulomb self energy is
+    (mn|nm) = 2 K^2 (pi/p)^3 sqrt(2p/pi) (Ry, e^2 = 2).  Sigma_x(m) =
+    -sum_n occ_n (mn|nm).  A dropped or doubled h^3, a wrong sign, an
+    unoccupied state in the sum (occ below tol_occ) or pair densities of the
+    wrong states fail it."""
+    from math import pi, sqrt
+
+    from nanogw_amd import synth
+
+    g = synth.sphere_grid(4.5, h)
+    _, nfft, trunc = synth.fft_box(g, h)
+    cen = np.array([[0.0, 0.0, 0.0], [0.9, 0.3, 0.0], [-0.4, -0.8, 0.5], [0.3, 0.6, -0.9], [0.0, 0.0, 1.0]])
+    alpha = np.array([1.6, 1.2, 2.0, 1.4, 1.1])
+    psi = _gaussian_orbitals(g, h, cen, alpha)
+    occ = np.array([1.0, 1.0, 0.6, 0.005, 0.0])  # the last two are not occupied (tol_occ = 0.01)
+    qp = [0, 3, 4]
+    got = O.sigma_x_direct(psi, occ, g, nfft, h, trunc, qp)
+    ref = []
+    for m in qp:
+        s = 0.0
+        for n in range(3):
+            p = alpha[m] + alpha[n]
+            nm, nn = (2 * alpha[m] / pi) ** 0.75, (2 * alpha[n] / pi) ** 0.75
+            k = nm * nn * np.exp(-alpha[m] * alpha[n] * ((cen[m] - cen[n]) ** 2).sum() / p)
+            s += occ[n] * 2.0 * k * k * (pi / p) ** 3 * sqrt(2.0 * p / pi)
+        ref.append(-s)
+    ref = np.array(ref)
+    print("sigma_x direct vs closed form", got, ref, np.abs(got / ref - 1).max())
+    assert np.abs(got / ref - 1.0).max() < 1e-7
+    # ntrans and nk enter as ntrans / nk
+    assert rel_err(O.sigma_x_direct(psi, occ, g, nfft, h, trunc, qp, ntrans=2, nk=4), 0.5 * got) < 1e-14
+
+
+def test_isdf_exchange_equals_direct_in_exact_limit(O):
+    """With every grid point an interpolation point (zeta = identity, so the
+    ISDF fit is exact), the ISDF exchange -sum_v C_v^T M C_v, C_v = psi_m psi_v
+    at the points and M = (1/h^3) zeta^T V zeta (synth.exchange_diag, the
+    Sigma_x input of both paths; the device Sigma_x is checked against it),
+    equals the reference's direct exchange (sigma_x_direct) to rounding."""
+    from nanogw_amd import synth
+
+    h = 0.4
+    g = synth.sphere_grid(2.4, h)
+    _, nfft, trunc = synth.fft_box(g, h)
+    rng = np.random.default_rng(17)
+    nv, nb = 3, 6
+    psi = rng.normal(size=(g.shape[0], nb))
+    psi = np.asfortranarray(psi / np.sqrt((psi ** 2).sum(axis=0)))
+    mm = O.isdf_kernel(np.eye(g.shape[0]), g, nfft, h, trunc)
+    qp = [0, 2, 3, 5]
+    isdf = synth.exchange_diag(psi, mm, nv, qp)
+    direct = O.sigma_x_direct(psi, [1.0] * nv + [0.0] * (nb - nv), g, nfft, h, trunc, qp)
+    print("exact-limit ISDF exchange vs direct", isdf, direct)
+    assert rel_err(isdf, direct) < 1e-11
+
+
 def test_cvt_lattice_lloyd_step_by_brute_force(O):
     """cvt_lattice (the CVT restatement the device CVT is checked against),
     pinned by a brute-force Lloyd step: from the restatement's points after
