@@ -64,3 +64,40 @@ def test_header_documents_reference_interfaces():
                 "printsigma.F90:97-128", "polyfit.f90:1", "isdf_parallel_sym_UltraLowMem.F90",
                 "fft_module.F90z:122-177", "apply_matvec.f90:25", "lanczos_spectra.f90:7"]:
         assert ref in text, ref
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/ngw.h compiles as strict C99 and as C++ (the boundary carries
+    only C types), and a C host that takes the address of every declared entry
+    point links against libngw.so and runs its host-only calls."""
+    import os
+    import shutil
+
+    from nanogw_amd import _lib
+
+    if shutil.which("gcc") is None or shutil.which("g++") is None:
+        pytest.skip("no host C compiler")
+    _lib.load()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    inc = os.path.join(root, "include")
+    names = _lib.declared_symbols()
+    src = ['#include <stdio.h>', '#include "ngw.h"', "typedef void (*entry)(void);", "int main(void) {",
+           "    entry fp[] = {" + ", ".join(f"(entry)&{n}" for n in names) + "};",
+           "    size_t i, n = sizeof fp / sizeof fp[0];",
+           "    for (i = 0; i < n; ++i) if (!fp[i]) return 2;",
+           '    printf("%s|%d|%s\\n", ngw_version(), ngw_synchronize(0), ngw_last_error());',
+           "    return 0;", "}"]
+    c = tmp_path / "host.c"
+    c.write_text("\n".join(src) + "\n")
+    exe = tmp_path / "host"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", f"-I{inc}", str(c), "-o", str(exe),
+                    f"-L{libdir}", "-lngw", f"-Wl,-rpath,{libdir}"], check=True, capture_output=True, text=True)
+    cpp = tmp_path / "host.cpp"
+    cpp.write_text('#include "ngw.h"\nint main() { return ngw_version() ? 0 : 1; }\n')
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", f"-I{inc}", "-fsyntax-only", str(cpp)], check=True,
+                   capture_output=True, text=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.strip()
+    ver, st, err = out.split("|", 2)
+    assert "gfx950" in ver
+    assert int(st) == -1 and err
