@@ -6,7 +6,8 @@ build, the .F90z file preprocessed twice as SIGMA/Makefile:107-110 does) and
 then nanogw_amd/fortran/ngw_dropin.F90 against them, so the component names,
 ranks and kinds the drop-in uses (ISDF%Psi_intp_loc, Mmtrx_loc, n_intp_r,
 PsiV_intp_bl, lvrep_bound, w_grp%npes, gvec%syms%prod, ...; typedefs.F90:24-72,
-mpi_module.F90z:20-85) are checked by the compiler.  A copy of the drop-in with
+mpi_module.F90z:20-85) are checked by the compiler, and the object exports the
+linker symbols NanoGW's drivers call.  A copy of the drop-in with
 one component name changed must then fail to compile against the same
 modules, which shows the check reads the real types.  Skipped where the
 reference sources or the Fortran compiler are absent."""
@@ -34,6 +35,13 @@ pytestmark = pytest.mark.skipif(not _have_reference(),
                                 reason="reference type modules or amdflang not available")
 
 
+# linker symbols NanoGW's drivers call (lower case, trailing underscore)
+SYMBOLS = ["matvec_isdf_block_", "polynomial_matvec_isdf_block_", "lanczos_spectra_isdf_block_",
+           "matvec_isdf_", "polynomial_matvec_isdf_", "lanczos_spectra_isdf_",
+           "matvec_isdf_ultralowmem_", "polynomial_matvec_isdf_ultralowmem_", "lanczos_spectra_isdf_ultralowmem_",
+           "matvec_isdf_lowcomsym_", "polynomial_matvec_isdf_lowcomsym_", "lanczos_spectra_isdf_lowcomsym_"]
+
+
 def _compile(src, outdir):
     return subprocess.run([FC, "-O2", "-cpp", f"-I{REFMODS}", f"-J{outdir}", "-c", src,
                            "-o", os.path.join(outdir, "dropin.o")],
@@ -48,6 +56,11 @@ def test_dropin_compiles_against_reference_types(tmp_path):
         assert os.path.exists(os.path.join(REFMODS, mod))
     # the unmodified drop-in compiles from a second directory too
     assert _compile(DROPIN, str(tmp_path)).returncode == 0
+    if shutil.which("nm"):
+        out = subprocess.run(["nm", os.path.join(REFMODS, "ngw_dropin.o")], capture_output=True, text=True,
+                             check=True).stdout
+        defined = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+        assert not [sym for sym in SYMBOLS if sym not in defined]
 
 
 @pytest.mark.parametrize("old,new", [("%Mmtrx_loc", "%Mmtrx_lox"), ("%lvrep_bound", "%lvrep_bnd")])
