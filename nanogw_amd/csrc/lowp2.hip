@@ -846,8 +846,9 @@ void k3v2_kernel(const LpOp* ops, LpOp op1, const LpStage* st, LpStage st1,
                 }
                 return;
             }
-            // two groups in flight where the tile runs at two waves per SIMD
-            // (its register budget has room), one at three waves per SIMD
+            // two groups in flight for the 128 x 192 tile (two waves per SIMD,
+            // with register room), one for 128 x 128 (three waves per SIMD)
+            // and for 128 x 256 (its accumulators leave no room)
             constexpr int GUNROLL = BM * BN <= 128 * 128 || BM * BN > 128 * 192 ? 1 : 2;
 #pragma unroll GUNROLL
             for (int k = 0; k < PER; ++k) {
