@@ -23,7 +23,10 @@
  *   orc_polynomial_matvec_block : Z p(Lambda) Z^T from numpy eigh.
  *   orc_sigma_lanczos         : sum-over-states with lambda_s = p(Omega_s^2)
  *                               at niter = ncv (Lanczos exact limit).
- *   orc_spline_evaluate / orc_newtonr / orc_qp_solve : a cubic table is
+ *   orc_spline_evaluate / orc_newtonr / orc_qp_solve : equal to a few ulps
+ *                               to the reference's own newtonr.f90 /
+ *                               spline_evaluate.f90 compiled from its sources
+ *                               (oracle/_ref); a cubic table is
  *                               reproduced to rounding away from the ends,
  *                               O(h^4) convergence on smooth tables, one-step
  *                               Newton on linear f(E), the closed-form root of

@@ -23,8 +23,10 @@ Pinning (tests/test_oracle_*.py):
     form (erf-form self energies of the Gaussian pair densities); in the
     exact-ISDF limit (every grid point an interpolation point) the ISDF
     exchange -sum_v C^T M C equals it to rounding;
-  * spline/Newton: exactness on a cubic table away from the ends (the end
-    slopes are finite differences), one-step Newton on linear f;
+  * spline/Newton: the reference's own newtonr / spline_evaluate compiled
+    from its sources (oracle/_ref/libngw_ref_spline.so, make target
+    ref-spline) agree to a few ulps; exactness on a cubic table away from the
+    ends (the end slopes are finite differences), one-step Newton on linear f;
   * polyfit: numpy.linalg.lstsq on well-conditioned cases.  The npoly = 12
     LAPACK dgesvx coefficients are parity unpinned; they are pinned inputs.
   * cvt_lattice: a brute-force Lloyd step (full distance matrix, weighted
@@ -57,13 +59,53 @@ def build() -> str:
 
 def build_ref() -> bool:
     """Where the reference sources are present, compile NanoGW's own Fortran
-    type modules into oracle/_ref/fortran and the drop-in against them
-    (make target dropin-types); False where they are absent."""
+    type modules into oracle/_ref/fortran and the drop-in against them (make
+    target dropin-types), and the reference's QP root finder into
+    oracle/_ref/libngw_ref_spline.so (target ref-spline); False where the
+    sources are absent."""
     r = subprocess.run(["make", "-s", "-C", _HERE, "ref-present"], capture_output=True, text=True)
     if r.stdout.strip() != "yes":
         return False
-    subprocess.run(["make", "-s", "-C", _HERE, "dropin-types"], check=True)
+    subprocess.run(["make", "-s", "-C", _HERE, "dropin-types", "ref-spline"], check=True)
     return True
+
+
+_REF = None
+
+
+def ref_lib():
+    """The reference's own newtonr / spline_evaluate (SIGMA/newtonr.f90,
+    spline_evaluate.f90, spline.f90, splint.f90 compiled as they are, by the
+    ref-spline make target), or None where that library was not built.
+    Loaded with lazy binding: its mpi_module objects name BLAS routines that
+    this path never calls."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libngw_ref_spline.so")
+        if not os.path.exists(path):
+            return None
+        _REF = C.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        _REF.newtonr_.restype = None
+        _REF.spline_evaluate_.restype = None
+    return _REF
+
+
+def ref_spline_evaluate(xtable, ftable, xx):
+    """spline_evaluate(ntable, xtable, ftable, xx, fx, df) of the reference."""
+    xtable, ftable = _f64(xtable), _c128(ftable)
+    n, x, df = C.c_int(len(xtable)), C.c_double(xx), C.c_double()
+    fx = np.zeros(1, np.complex128)
+    ref_lib().spline_evaluate_(C.byref(n), _p(xtable), _p(ftable), C.byref(x), _p(fx), C.byref(df))
+    return complex(fx[0]), df.value
+
+
+def ref_newtonr(xtable, ftable, x0, xacc=0.01):
+    """newtonr(ntable, xtable, ftable, x_zero, xacc, fx, ierr) of the reference."""
+    xtable, ftable = _f64(xtable), _c128(ftable)
+    n, x, xa, ierr = C.c_int(len(xtable)), C.c_double(x0), C.c_double(xacc), C.c_int()
+    fx = np.zeros(1, np.complex128)
+    ref_lib().newtonr_(C.byref(n), _p(xtable), _p(ftable), C.byref(x), C.byref(xa), _p(fx), C.byref(ierr))
+    return x.value, complex(fx[0]), ierr.value
 
 
 def lib():

@@ -243,6 +243,36 @@ def test_spline_cubic_interior(O):
         assert abs(fx - (np.sin(0.246) + 1j * np.cos(0.123))) < tol
 
 
+@pytest.mark.parametrize("n", [5, 21, 201])
+def test_spline_and_newton_equal_the_reference_build(O, n):
+    """The oracle's spline_evaluate / newtonr against the reference's own
+    SIGMA/spline_evaluate.f90, spline.f90, splint.f90 and newtonr.f90,
+    compiled from their sources into oracle/_ref (make -C oracle ref-spline):
+    interpolated value and slope on uneven complex tables, and the Newton
+    root, residual and error flag, including a table without a root (the
+    reference returns ierr = MAX_IT + 1 = 21).  Skipped where that library
+    was not built (no reference sources)."""
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref/libngw_ref_spline.so not built (reference sources absent)")
+    rng = np.random.default_rng(60 + n)
+    x = np.sort(np.concatenate([[-2.0, 3.0], rng.uniform(-2.0, 3.0, n - 2)]))
+    f = np.sin(1.3 * x) + 0.2 * x - 0.1 + 1j * (0.3 * np.cos(x) + 0.05 * rng.normal(size=n))
+    eps = np.finfo(float).eps
+    for xx in np.concatenate([rng.uniform(x[1], x[-2], 6), [x[2], x[-3]]]):
+        fa, da = O.spline_evaluate(x, f, xx)
+        fb, db = O.ref_spline_evaluate(x, f, xx)
+        assert abs(fa - fb) <= 8 * eps * max(1.0, abs(fb)), (xx, fa, fb)
+        assert abs(da - db) <= 1e3 * eps * max(1.0, abs(db)), (xx, da, db)
+    for x0, acc in ((0.1, 1e-6), (-1.5, 1e-3)):
+        ra, rb = O.newtonr(x, f, x0, acc), O.ref_newtonr(x, f, x0, acc)
+        assert ra[2] == rb[2]
+        assert abs(ra[0] - rb[0]) <= 1e-12 * max(1.0, abs(rb[0]))
+        assert abs(ra[1] - rb[1]) <= 1e-12 * max(1.0, abs(rb[1]))
+    g = 2.0 + 0.1 * x ** 2 + 0.0j  # no real root
+    ra, rb = O.newtonr(x, g, 0.3, 1e-8), O.ref_newtonr(x, g, 0.3, 1e-8)
+    assert rb[2] == 21 and ra[2] == rb[2]
+
+
 def test_spline_exact_on_cubic_table_interior(O):
     """spline.f90 takes its boundary slopes from the end pairs, so a cubic
     table is not reproduced near the ends; the boundary error decays by about
