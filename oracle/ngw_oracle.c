@@ -248,7 +248,7 @@ static void lanczos_block_generic(orc_apply_fn apply, void *ctx, const double *v
                                   int nz, int niter, double complex *spectra, int blksz,
                                   double *a_out, double *b_out)
 {
-    double *vec = (double *)malloc(sizeof(double) * ncv * blksz);
+    double *vec = (double *)calloc((size_t)ncv * blksz, sizeof(double));
     double *wvec = (double *)malloc(sizeof(double) * ncv * blksz);
     double *tmp = (double *)malloc(sizeof(double) * ncv * blksz);
     double *a = (double *)calloc((size_t)niter * blksz, sizeof(double));
@@ -379,7 +379,7 @@ void orc_lanczos_spectra_explicit(int nmu, long ncv, const double *cm, long ldc,
                                   double complex *spectra, double *a_out, double *b_out, int *nk)
 {
     const double eps = 1.0e-8;
-    double *vec = (double *)malloc(sizeof(double) * ncv);
+    double *vec = (double *)calloc((size_t)ncv, sizeof(double));
     double *wvec = (double *)malloc(sizeof(double) * ncv);
     double *tmp = (double *)malloc(sizeof(double) * ncv);
     double *a = (double *)calloc((size_t)niter, sizeof(double));
@@ -636,7 +636,7 @@ void orc_spline_evaluate(int ntable, const double *xtable, const double complex 
     int nn = ntable - 2;
     double complex yp1 = (ftable[1] - ftable[0]) / (xtable[1] - xtable[0]);
     double complex ypn = (ftable[ntable - 1] - ftable[ntable - 2]) / (xtable[ntable - 1] - xtable[ntable - 2]);
-    double *yr = (double *)malloc(sizeof(double) * nn), *yi = (double *)malloc(sizeof(double) * nn);
+    double *yr = (double *)calloc((size_t)nn, sizeof(double)), *yi = (double *)calloc((size_t)nn, sizeof(double));
     double *y2r = (double *)malloc(sizeof(double) * nn), *y2i = (double *)malloc(sizeof(double) * nn);
     double complex *y2 = (double complex *)malloc(sizeof(double complex) * nn);
     for (int i = 0; i < nn; ++i) { yr[i] = creal(ftable[i + 1]); yi[i] = cimag(ftable[i + 1]); }
