@@ -62,7 +62,9 @@ def build_ref() -> bool:
     type modules into oracle/_ref/fortran and the drop-in against them (make
     target dropin-types), and the reference's QP root finder into
     oracle/_ref/libngw_ref_spline.so (target ref-spline); False where the
-    sources are absent."""
+    sources or the Fortran compiler are absent."""
+    if not os.path.exists(os.environ.get("NGW_FC", "/opt/rocm/bin/amdflang")):
+        return False
     r = subprocess.run(["make", "-s", "-C", _HERE, "ref-present"], capture_output=True, text=True)
     if r.stdout.strip() != "yes":
         return False
