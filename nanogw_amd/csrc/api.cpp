@@ -124,7 +124,7 @@ static int upload(ngw_context* ctx, DevBuf& buf, const void* src, size_t bytes)
 
 // ------------------------------------------------------------ host polyfit
 // Equilibrated LU solve with refinement, the algorithm of LAPACK dgesvx with
-// fact = 'E' that shared/linear_solver.f90:214 calls.
+// fact = 'E' that shared/linear_solver.f90:20-28 calls.
 static void solve_equilibrated(int n, const std::vector<double>& Ain, const std::vector<double>& bin,
                                double* x)
 {

@@ -713,7 +713,7 @@ int orc_qp_solve(int nen, double deltae_ev, double edft, double en0, double sigx
  * Solve A x = b (n x n) the way LAPACK dgesvx(fact='E') does it: dgeequ
  * row/column equilibration applied only when dlaqge's thresholds call for it,
  * LU with partial pivoting, dgerfs-style iterative refinement (<= 5 steps),
- * then unscaling (shared/linear_solver.f90:206-215).
+ * then unscaling (the dgesvx call of shared/linear_solver.f90:20-28).
  */
 static void solve_equilibrated(int n, const double *Ain, const double *bin, double *x)
 {
